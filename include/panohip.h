@@ -1,0 +1,178 @@
+/*
+ * panohip.h -- C-ABI of the MI355X (gfx950) per-frame panorama stitch path.
+ *
+ * A drop-in for PanoStitcher's CudaAccel layer on the path
+ *     per-camera gain LUTs -> bilinear remap through float32 maps ->
+ *     integer Gaussian/Laplacian multiband blend -> 8-bit BGRx canvas.
+ * Results follow the reference's CPU path (CPUPanoramaRender), which is the
+ * parity target: every step is integer and bit-exact except the FP64
+ * bilinear sample, which is evaluated in the same order without contraction.
+ *
+ * Conventions
+ *   - every entry point returns an int status: PANO_OK (0) or a negative code;
+ *     pano_status_string() names it.  Nothing throws across this boundary.
+ *   - device images are pitched: a device pointer plus a row step in bytes.
+ *     Colour images are 4-channel BGRx (CV_8UC4 / CV_16SC4), as in the
+ *     reference's GPU API; masks are single-channel bytes (0 / non-zero).
+ *   - "host" arguments are read during the call only; "device" arguments must
+ *     stay valid until the work queued on the given stream has finished.
+ *   - stream: any hipStream_t (0 = the null stream).  Work of one object is
+ *     queued in order on the stream passed to each call; one object must not
+ *     be used from two threads at once (the reference classes are not
+ *     thread-safe either, CudaMultibandBlend.cpp:785-823 keeps per-call
+ *     scratch in members).
+ */
+#ifndef PANOHIP_H
+#define PANOHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define PANO_OK 0
+#define PANO_ERR_ARG (-1)         /* bad argument: sizes, types, null pointers */
+#define PANO_ERR_STATE (-2)       /* object not prepared / prepare failed       */
+#define PANO_ERR_HIP (-3)         /* a HIP runtime call failed                  */
+#define PANO_ERR_NOMEM (-4)       /* device or host allocation failed           */
+#define PANO_ERR_UNSUPPORTED (-5) /* valid request this build does not handle   */
+#define PANO_ERR_BUDGET (-6)      /* admission refused: device memory budget    */
+
+/* ---- image type codes (OpenCV CV_MAKETYPE values) ---------------------- */
+#define PANO_8UC1 0
+#define PANO_8UC3 16
+#define PANO_8UC4 24
+#define PANO_16SC4 27
+
+const char* pano_status_string(int status);
+int pano_version(void);
+
+/* ---- camera model: PhotoParam (Warp/ZReproject.h:17-50), same fields ---- */
+typedef struct pano_photo_param {
+    int imageType; /* 0 rectilinear, 1 full-frame fisheye, 2 drum, 3 circular */
+    int cropMode, cropX, cropY, cropWidth, cropHeight;
+    int circleX, circleY, circleR;
+    double hfov, vfov, alpha, beta, gamma, shiftX, shiftY, shearX, shearY;
+    double yaw, pitch, roll;
+} pano_photo_param;
+
+/* ---- multiband blender: CudaTilingMultibandBlendFast -------------------
+ * Replaces the class in CudaAccel/CudaInterface.h:45-82 (implementation
+ * CudaAccel/CudaMultibandBlend.cpp:549-823), with the arithmetic of the CPU
+ * TilingMultibandBlendFast (Blend/MultiBandBlend.cpp:906-1031). */
+typedef struct pano_blender pano_blender;
+
+int pano_blender_create(pano_blender** out);
+void pano_blender_destroy(pano_blender* b);
+
+/* prepare(const vector<Mat>& masks, int maxLevels, int minLength)
+ * (CudaInterface.h:49).  masks: host, num planes of rows x cols bytes with
+ * the given row steps.  Fails (PANO_ERR_ARG) on empty or mismatched masks or
+ * more than 255 masks, like CudaMultibandBlend.cpp:551-565. */
+int pano_blender_prepare(pano_blender* b, int num, int rows, int cols,
+                         const uint8_t* const* masks, const size_t* mask_steps, int max_levels,
+                         int min_length);
+
+/* prepare(const vector<GpuMat>& masks, ...) (CudaInterface.h:50): the same with
+ * device masks; the masks are read back and prepared on the host. */
+int pano_blender_prepare_device(pano_blender* b, int num, int rows, int cols,
+                                const uint8_t* const* masks, const size_t* mask_steps,
+                                int max_levels, int min_length);
+
+/* blend(const vector<GpuMat>& images, GpuMat& blendImage) (CudaInterface.h:54).
+ * images: device, num images of rows x cols, type PANO_16SC4 or PANO_8UC4.
+ * out: device rows x cols PANO_8UC4 (channel 3 written as 0). */
+int pano_blender_blend(pano_blender* b, int num, const void* const* images, const size_t* steps,
+                       int type, uint8_t* out, size_t out_step, hipStream_t stream);
+
+/* begin() / tile(image, index) / end(blendImage) (CudaInterface.h:51-53).
+ * tile() consumes the image before it returns on the stream: the caller may
+ * reuse the buffer for the next camera once the stream reaches that point. */
+int pano_blender_begin(pano_blender* b);
+int pano_blender_tile(pano_blender* b, const void* image, size_t step, int type, int index,
+                      hipStream_t stream);
+int pano_blender_end(pano_blender* b, uint8_t* out, size_t out_step, hipStream_t stream);
+
+/* getUniqueMasks (CudaInterface.h:57): host output, num planes of rows x cols. */
+int pano_blender_unique_masks(const pano_blender* b, uint8_t* const* out, const size_t* steps);
+int pano_blender_num_levels(const pano_blender* b);
+/* calcUsedMemorySize (CudaInterface.h:58) and estimateMemorySize (:59). */
+int64_t pano_blender_mem_bytes(const pano_blender* b);
+int64_t pano_estimate_mem_bytes(int width, int height, int num);
+
+/* ---- free functions ----------------------------------------------------- */
+
+/* cudaReprojectTo16S(src, dst, xmap, ymap, stream) (CudaInterface.h:150-151),
+ * with the CPU path's bilinear arithmetic (Warp/ZReproject.cpp:101-130) rather
+ * than the reference GPU's bicubic texture fetch.  src: device PANO_8UC4;
+ * xmap/ymap: device float32 planes of dst_w x dst_h (invalid = -1); dst:
+ * device PANO_16SC4 dst_w x dst_h, zero where the map is invalid. */
+int pano_reproject_to16s(const uint8_t* src, size_t src_step, int src_w, int src_h,
+                         const float* xmap, const float* ymap, size_t map_step, int16_t* dst,
+                         size_t dst_step, int dst_w, int dst_h, hipStream_t stream);
+
+/* cudaTransform(src, dst, luts, stream) (CudaInterface.h:120-121):
+ * per-channel LUTs on BGRx (channel 3 copied).  luts: host, 3 x 256 bytes
+ * (B, G, R).  src and dst may alias. */
+int pano_transform_lut3(const uint8_t* src, size_t src_step, uint8_t* dst, size_t dst_step, int w,
+                        int h, const uint8_t* luts, hipStream_t stream);
+
+/* getLUT (Blend/ExposureColorOptimize.cpp:1217-1223): lut[v] = sat_u8(rint(k v)). */
+int pano_get_lut(uint8_t* lut, double k);
+/* getExposureColorOptimizeLUTs (ExposureColorOptimize.cpp:1425-1443) without
+ * the max-scale clamp the reference computes but does not apply:
+ * luts[i] = {getLUT(e b), getLUT(e), getLUT(e r)}, n x 3 x 256 bytes. */
+int pano_exposure_luts(int num, const double* exposures, const double* red_ratios,
+                       const double* blue_ratios, uint8_t* luts);
+
+/* getReprojectMap32FAndMask (Warp/ZReproject.cpp:231-240) on the host, rows
+ * in parallel: xmap/ymap float32 dst_w x dst_h (-1 where invalid), mask 0/255. */
+int pano_generate_map(const pano_photo_param* param, int src_w, int src_h, int dst_w, int dst_h,
+                      float* xmap, float* ymap, uint8_t* mask);
+
+/* ---- render: CudaPanoramaRender2 (Task/RicohUtil.h:134-161) -------------
+ * prepare + per-frame render of one rig: maps, masks, seams and weight
+ * pyramids stay resident on the device; render() runs the fused path
+ * (LUT + remap -> pyramid -> blend) with one launch per stage for all cameras.
+ * Cameras may have different source sizes. */
+typedef struct pano_render pano_render;
+
+int pano_render_create(pano_render** out);
+void pano_render_destroy(pano_render* r);
+
+/* prepare(path, highQualityBlend = 1, blendParam = maxLevels, srcSize, dstSize)
+ * (RicohUtil.cpp:1091-1205) with the parameters given directly instead of a
+ * rig file.  dst_w must equal 2 * dst_h, both even (RicohUtil.cpp:1098-1103). */
+int pano_render_prepare(pano_render* r, int num, const pano_photo_param* params, const int* src_w,
+                        const int* src_h, int dst_w, int dst_h, int max_levels);
+
+/* The same with caller-made maps and masks (host float32 planes dst_w x dst_h
+ * and host masks), the inputs getReprojectMaps32FAndMasks would produce. */
+int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const int* src_h,
+                             int dst_w, int dst_h, const float* const* xmaps,
+                             const float* const* ymaps, const uint8_t* const* masks,
+                             int max_levels);
+
+/* render(src, dst, luts) (RicohUtil.cpp:1207-1339), multiband branch.
+ * srcs: device PANO_8UC4 images (sizes as prepared).  luts: host, num x 3 x
+ * 256 bytes, or NULL for no correction.  dst: device PANO_8UC4 canvas. */
+int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
+                       const size_t* src_steps, const uint8_t* luts, uint8_t* dst,
+                       size_t dst_step, hipStream_t stream);
+
+int pano_render_num_images(const pano_render* r);
+int pano_render_num_levels(const pano_render* r);
+int64_t pano_render_mem_bytes(const pano_render* r);
+/* Algorithmic HBM bytes of one render() (inputs read once, outputs written
+ * once, per stage), and of each stage kernel; see DESIGN notes in the source. */
+int64_t pano_render_model_bytes(const pano_render* r, int stage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

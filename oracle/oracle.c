@@ -535,10 +535,13 @@ int orc_blender_alpha(const orc_blender* b, int cam, int level, int32_t* out)
 /*
  * bilinearResampling<short, cn> (ZReproject.cpp:101-130) for one valid map
  * point.  x0 = (int)x, y1 = min(y0 + 1, h - 1); the second sample of each row
- * is the next pixel in memory (the reference advances its pointer past x0 and
- * never uses its clamped x1), except at the last pixel of the buffer, where
- * the reference would read past its end: there the pixel itself is used.
+ * is the next pixel in row-major order (the reference advances its pointer
+ * past x0 and never uses its clamped x1, ZReproject.cpp:117-126, on images
+ * that are continuous -- transform() allocates a continuous image): at
+ * x0 = w - 1 that is pixel (0, y + 1); at the last pixel of the image, where
+ * the reference would read past its buffer, the pixel itself is used.
  * Sum order p00 w00, p01 w01, p10 w10, p11 w11; saturate_cast<short> = cvRound.
+ * Callers here always pass continuous images (sstep == sw * cn).
  */
 static void orc_bilinear_s16(const uint8_t* src, int sw, int sh, size_t sstep, int cn, double x,
                              double y, int16_t* d)

@@ -1,0 +1,392 @@
+"""panostitcher_amd -- MI355X (gfx950) per-frame panorama stitch path.
+
+Python mirror of the reference's CudaAccel interface for this path, over the
+C-ABI library ``libpanohip.so`` (include/panohip.h).  Names and argument order
+follow CudaAccel/CudaInterface.h and Task/RicohUtil.h:
+
+  * ``CudaTilingMultibandBlendFast``: prepare(masks, max_levels, min_length),
+    blend(images, out), begin()/tile(image, index)/end(out), get_unique_masks(),
+    calc_used_memory_size(), estimate_memory_size(w, h, n);
+  * ``cuda_reproject_to16s(src, xmap, ymap, stream)``  (bilinear, CPU arithmetic);
+  * ``cuda_transform(src, luts, stream)``;
+  * ``CudaPanoramaRender2``-style ``PanoramaRender``: prepare(params | maps+masks)
+    and render(srcs, luts) -> BGRx canvas.
+
+Device memory is torch tensors on ``cuda`` (ROCm); streams are torch streams.
+Errors: prepare() returns False like the reference; a render/blend on an
+unprepared object returns False; invalid arguments raise ValueError (the
+reference's CV_Assert throws).  The native library is required: importing the
+wrappers without it raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libpanohip.so")
+
+PANO_OK = 0
+PANO_8UC4 = 24
+PANO_16SC4 = 27
+
+_lib = None
+
+
+def build(arch: str = "gfx950", jobs: int = 8) -> str:
+    """Compile libpanohip.so in-tree with hipcc (cross-compiles without a GPU)."""
+    import subprocess
+
+    subprocess.run(["make", "-s", "-C", _HERE, f"-j{jobs}", f"ARCH={arch}", "libpanohip.so"],
+                   check=True)
+    return LIB_PATH
+
+
+class PanoError(RuntimeError):
+    pass
+
+
+def lib():
+    """The loaded native library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PanoError(f"{LIB_PATH} is missing: run panostitcher_amd.build() first")
+        L = C.CDLL(LIB_PATH)
+        vp, i, sz = C.c_void_p, C.c_int, C.c_size_t
+        i64 = C.c_int64
+        pp = C.POINTER(C.c_void_p)
+        psz = C.POINTER(C.c_size_t)
+        sigs = {
+            "pano_status_string": ([i], C.c_char_p),
+            "pano_version": ([], i),
+            "pano_blender_create": ([pp], i),
+            "pano_blender_destroy": ([vp], None),
+            "pano_blender_prepare": ([vp, i, i, i, pp, psz, i, i], i),
+            "pano_blender_prepare_device": ([vp, i, i, i, pp, psz, i, i], i),
+            "pano_blender_blend": ([vp, i, pp, psz, i, vp, sz, vp], i),
+            "pano_blender_begin": ([vp], i),
+            "pano_blender_tile": ([vp, vp, sz, i, i, vp], i),
+            "pano_blender_end": ([vp, vp, sz, vp], i),
+            "pano_blender_unique_masks": ([vp, pp, psz], i),
+            "pano_blender_num_levels": ([vp], i),
+            "pano_blender_mem_bytes": ([vp], i64),
+            "pano_estimate_mem_bytes": ([i, i, i], i64),
+            "pano_reproject_to16s": ([vp, sz, i, i, vp, vp, sz, vp, sz, i, i, vp], i),
+            "pano_transform_lut3": ([vp, sz, vp, sz, i, i, vp, vp], i),
+            "pano_get_lut": ([vp, C.c_double], i),
+            "pano_exposure_luts": ([i, vp, vp, vp, vp], i),
+            "pano_generate_map": ([vp, i, i, i, i, vp, vp, vp], i),
+            "pano_render_create": ([pp], i),
+            "pano_render_destroy": ([vp], None),
+            "pano_render_prepare": ([vp, i, vp, vp, vp, i, i, i], i),
+            "pano_render_prepare_maps": ([vp, i, vp, vp, i, i, pp, pp, pp, i], i),
+            "pano_render_render": ([vp, i, pp, psz, vp, vp, sz, vp], i),
+            "pano_render_num_images": ([vp], i),
+            "pano_render_num_levels": ([vp], i),
+            "pano_render_mem_bytes": ([vp], i64),
+            "pano_render_model_bytes": ([vp, i], i64),
+        }
+        for name, (args, res) in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return [
+        "pano_status_string", "pano_version", "pano_blender_create", "pano_blender_destroy",
+        "pano_blender_prepare", "pano_blender_prepare_device", "pano_blender_blend",
+        "pano_blender_begin", "pano_blender_tile", "pano_blender_end",
+        "pano_blender_unique_masks", "pano_blender_num_levels", "pano_blender_mem_bytes",
+        "pano_estimate_mem_bytes", "pano_reproject_to16s", "pano_transform_lut3", "pano_get_lut",
+        "pano_exposure_luts", "pano_generate_map", "pano_render_create", "pano_render_destroy",
+        "pano_render_prepare", "pano_render_prepare_maps", "pano_render_render",
+        "pano_render_num_images", "pano_render_num_levels", "pano_render_mem_bytes",
+        "pano_render_model_bytes",
+    ]
+
+
+def status_string(st: int) -> str:
+    return lib().pano_status_string(st).decode()
+
+
+def _check(st: int, what: str):
+    if st != PANO_OK:
+        raise PanoError(f"{what}: {status_string(st)} ({st})")
+
+
+def _ptrs(arrs):
+    return (C.c_void_p * len(arrs))(*[C.c_void_p(a) for a in arrs])
+
+
+def _sizes(vals):
+    return (C.c_size_t * len(vals))(*vals)
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+
+        stream = torch.cuda.current_stream()
+    return C.c_void_p(stream.cuda_stream)
+
+
+def _dev_ptr_step(t):
+    """(pointer, row step in bytes) of a torch tensor HxWxC (rows may be padded)."""
+    assert t.is_cuda, "device tensor expected"
+    return t.data_ptr(), t.stride(0) * t.element_size()
+
+
+# ---------------------------------------------------------------- photo params --
+
+class PhotoParam(C.Structure):
+    """PhotoParam (Warp/ZReproject.h:17-50) as pano_photo_param."""
+    _fields_ = [(n, C.c_int) for n in ("imageType", "cropMode", "cropX", "cropY", "cropWidth",
+                                        "cropHeight", "circleX", "circleY", "circleR")] + \
+               [(n, C.c_double) for n in ("hfov", "vfov", "alpha", "beta", "gamma", "shiftX",
+                                           "shiftY", "shearX", "shearY", "yaw", "pitch", "roll")]
+
+    @classmethod
+    def make(cls, **kw):
+        p = cls()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+def get_lut(k: float) -> np.ndarray:
+    out = np.zeros(256, np.uint8)
+    _check(lib().pano_get_lut(out.ctypes.data, float(k)), "getLUT")
+    return out
+
+
+def exposure_luts(exposures, red_ratios, blue_ratios) -> np.ndarray:
+    e = np.ascontiguousarray(exposures, np.float64)
+    r = np.ascontiguousarray(red_ratios, np.float64)
+    b = np.ascontiguousarray(blue_ratios, np.float64)
+    out = np.zeros((len(e), 3, 256), np.uint8)
+    _check(lib().pano_exposure_luts(len(e), e.ctypes.data, r.ctypes.data, b.ctypes.data,
+                                    out.ctypes.data), "getExposureColorOptimizeLUTs")
+    return out
+
+
+def generate_map(param: PhotoParam, src_w: int, src_h: int, dst_w: int, dst_h: int):
+    """getReprojectMap32FAndMask on the host: (xmap, ymap, mask)."""
+    xm = np.zeros((dst_h, dst_w), np.float32)
+    ym = np.zeros((dst_h, dst_w), np.float32)
+    mk = np.zeros((dst_h, dst_w), np.uint8)
+    _check(lib().pano_generate_map(C.byref(param), src_w, src_h, dst_w, dst_h, xm.ctypes.data,
+                                   ym.ctypes.data, mk.ctypes.data), "generate map")
+    return xm, ym, mk
+
+
+# ------------------------------------------------------------------- blender --
+
+class CudaTilingMultibandBlendFast:
+    """CudaTilingMultibandBlendFast (CudaAccel/CudaInterface.h:45-82)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().pano_blender_create(C.byref(h)), "create blender")
+        self._h = h
+        self.num_images = 0
+        self.rows = self.cols = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pano_blender_destroy(h)
+            self._h = None
+
+    def prepare(self, masks, max_levels: int, min_length: int = 2) -> bool:
+        """masks: list of HxW uint8 host arrays (or torch cuda tensors)."""
+        if len(masks) == 0:
+            return False
+        dev = hasattr(masks[0], "is_cuda") and masks[0].is_cuda
+        rows, cols = masks[0].shape[:2]
+        if dev:
+            ptrs = [m.data_ptr() for m in masks]
+            steps = [m.stride(0) for m in masks]
+            st = lib().pano_blender_prepare_device(self._h, len(masks), rows, cols, _ptrs(ptrs),
+                                                   _sizes(steps), max_levels, min_length)
+        else:
+            arrs = [np.ascontiguousarray(m, np.uint8) for m in masks]
+            for a in arrs:
+                if a.shape != (rows, cols):
+                    return False
+            st = lib().pano_blender_prepare(self._h, len(arrs), rows, cols,
+                                            _ptrs([a.ctypes.data for a in arrs]),
+                                            _sizes([a.strides[0] for a in arrs]), max_levels,
+                                            min_length)
+        if st == PANO_OK:
+            self.num_images, self.rows, self.cols = len(masks), rows, cols
+        return st == PANO_OK
+
+    @property
+    def num_levels(self):
+        return lib().pano_blender_num_levels(self._h)
+
+    def _type(self, t):
+        import torch
+
+        if t.dtype == torch.int16 and t.shape[-1] == 4:
+            return PANO_16SC4, 8
+        if t.dtype == torch.uint8 and t.shape[-1] == 4:
+            return PANO_8UC4, 4
+        raise ValueError("images must be HxWx4 int16 (CV_16SC4) or uint8 (CV_8UC4)")
+
+    def blend(self, images, out, stream=None) -> bool:
+        """images: list of HxWx4 cuda tensors; out: HxWx4 uint8 cuda tensor."""
+        typ, _ = self._type(images[0])
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in images])
+        op, ostep = _dev_ptr_step(out)
+        st = lib().pano_blender_blend(self._h, len(images), _ptrs(ptrs), _sizes(steps), typ,
+                                      C.c_void_p(op), ostep, _stream(stream))
+        if st == -2:
+            return False
+        _check(st, "blend")
+        return True
+
+    def begin(self):
+        return lib().pano_blender_begin(self._h) == PANO_OK
+
+    def tile(self, image, index: int, stream=None):
+        typ, _ = self._type(image)
+        p, s = _dev_ptr_step(image)
+        _check(lib().pano_blender_tile(self._h, C.c_void_p(p), s, typ, index, _stream(stream)),
+               "tile")
+
+    def end(self, out, stream=None):
+        p, s = _dev_ptr_step(out)
+        _check(lib().pano_blender_end(self._h, C.c_void_p(p), s, _stream(stream)), "end")
+
+    def get_unique_masks(self):
+        outs = [np.zeros((self.rows, self.cols), np.uint8) for _ in range(self.num_images)]
+        _check(lib().pano_blender_unique_masks(self._h, _ptrs([o.ctypes.data for o in outs]),
+                                               _sizes([self.cols] * self.num_images)),
+               "getUniqueMasks")
+        return outs
+
+    def calc_used_memory_size(self) -> int:
+        return lib().pano_blender_mem_bytes(self._h)
+
+    @staticmethod
+    def estimate_memory_size(width: int, height: int, num_images: int) -> int:
+        return lib().pano_estimate_mem_bytes(width, height, num_images)
+
+
+# ------------------------------------------------------------- free functions --
+
+def cuda_reproject_to16s(src, xmap, ymap, dst=None, stream=None):
+    """cudaReprojectTo16S(src 8UC4, dst 16SC4, xmap, ymap) with bilinear sampling."""
+    import torch
+
+    sh, sw = src.shape[:2]
+    dh, dw = xmap.shape
+    if dst is None:
+        dst = torch.empty((dh, dw, 4), dtype=torch.int16, device=src.device)
+    sp, ss = _dev_ptr_step(src)
+    dp, ds = _dev_ptr_step(dst)
+    assert xmap.stride(0) == ymap.stride(0)
+    _check(lib().pano_reproject_to16s(C.c_void_p(sp), ss, sw, sh, C.c_void_p(xmap.data_ptr()),
+                                      C.c_void_p(ymap.data_ptr()), xmap.stride(0) * 4,
+                                      C.c_void_p(dp), ds, dw, dh, _stream(stream)),
+           "cudaReprojectTo16S")
+    return dst
+
+
+def cuda_transform(src, luts, dst=None, stream=None):
+    """cudaTransform(src 8UC4, dst, luts 3x256)."""
+    import torch
+
+    h, w = src.shape[:2]
+    if dst is None:
+        dst = torch.empty_like(src)
+    luts = np.ascontiguousarray(luts, np.uint8)
+    assert luts.shape == (3, 256)
+    sp, ss = _dev_ptr_step(src)
+    dp, ds = _dev_ptr_step(dst)
+    _check(lib().pano_transform_lut3(C.c_void_p(sp), ss, C.c_void_p(dp), ds, w, h,
+                                     luts.ctypes.data, _stream(stream)), "cudaTransform")
+    return dst
+
+
+# -------------------------------------------------------------------- render --
+
+class PanoramaRender:
+    """CudaPanoramaRender2 (Task/RicohUtil.h:134-161), multiband branch: maps,
+    seams and weights resident on the device; render() = LUT + remap + blend."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().pano_render_create(C.byref(h)), "create render")
+        self._h = h
+        self.num_images = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pano_render_destroy(h)
+            self._h = None
+
+    def prepare(self, params, src_sizes, dst_size, max_levels: int) -> bool:
+        n = len(params)
+        arr = (PhotoParam * n)(*params)
+        sw = np.array([s[0] for s in src_sizes], np.int32)
+        shh = np.array([s[1] for s in src_sizes], np.int32)
+        st = lib().pano_render_prepare(self._h, n, C.cast(arr, C.c_void_p), sw.ctypes.data,
+                                       shh.ctypes.data, dst_size[0], dst_size[1], max_levels)
+        if st == PANO_OK:
+            self.num_images, self.dst_size = n, tuple(dst_size)
+        return st == PANO_OK
+
+    def prepare_maps(self, src_sizes, dst_size, xmaps, ymaps, masks, max_levels: int) -> bool:
+        n = len(xmaps)
+        xs = [np.ascontiguousarray(x, np.float32) for x in xmaps]
+        ys = [np.ascontiguousarray(y, np.float32) for y in ymaps]
+        ms = [np.ascontiguousarray(m, np.uint8) for m in masks]
+        sw = np.array([s[0] for s in src_sizes], np.int32)
+        shh = np.array([s[1] for s in src_sizes], np.int32)
+        st = lib().pano_render_prepare_maps(self._h, n, sw.ctypes.data, shh.ctypes.data,
+                                            dst_size[0], dst_size[1],
+                                            _ptrs([x.ctypes.data for x in xs]),
+                                            _ptrs([y.ctypes.data for y in ys]),
+                                            _ptrs([m.ctypes.data for m in ms]), max_levels)
+        if st == PANO_OK:
+            self.num_images, self.dst_size = n, tuple(dst_size)
+        return st == PANO_OK
+
+    @property
+    def num_levels(self):
+        return lib().pano_render_num_levels(self._h)
+
+    def mem_bytes(self) -> int:
+        return lib().pano_render_mem_bytes(self._h)
+
+    def model_bytes(self, stage: int = 4) -> int:
+        return lib().pano_render_model_bytes(self._h, stage)
+
+    def render(self, srcs, out, luts=None, stream=None) -> bool:
+        """srcs: list of HxWx4 uint8 cuda tensors (BGRx); out: HxWx4 uint8 cuda tensor;
+        luts: None or (n, 3, 256) uint8 host array."""
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in srcs])
+        op, ostep = _dev_ptr_step(out)
+        lp = None
+        if luts is not None:
+            luts = np.ascontiguousarray(luts, np.uint8)
+            assert luts.shape == (len(srcs), 3, 256)
+            lp = C.c_void_p(luts.ctypes.data)
+        st = lib().pano_render_render(self._h, len(srcs), _ptrs(ptrs), _sizes(steps), lp,
+                                      C.c_void_p(op), ostep, _stream(stream))
+        if st == -2:
+            return False
+        _check(st, "render")
+        return True

@@ -1,0 +1,741 @@
+// capi.cpp -- the extern "C" boundary (include/panohip.h): object lifetimes,
+// device residency of prepared rigs, and the per-frame launch sequences.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <memory>
+#include <vector>
+
+#include "../../include/panohip.h"
+#include "pano_host.hpp"
+#include "pano_launch.hpp"
+
+namespace pano {
+int generate_map(const pano_photo_param& p, int src_w, int src_h, int dst_w, int dst_h,
+                 float* xmap, float* ymap, uint8_t* mask);
+}
+
+using namespace pano;
+
+#define HIP_TRY(expr)                                        \
+    do {                                                     \
+        hipError_t e_ = (expr);                              \
+        if (e_ != hipSuccess) return PANO_ERR_HIP;           \
+    } while (0)
+
+namespace {
+
+// One device allocation carved into 256-byte aligned pieces.
+struct Arena {
+    Arena() = default;
+    Arena(const Arena&) = delete;
+    Arena& operator=(const Arena&) = delete;
+    uint8_t* base = nullptr;
+    size_t size = 0, used = 0;
+    ~Arena() { release(); }
+    void release()
+    {
+        if (base) (void)hipFree(base);
+        base = nullptr;
+        size = used = 0;
+    }
+    static size_t align(size_t n) { return (n + 255) & ~(size_t)255; }
+    int reserve(size_t bytes)
+    {
+        release();
+        if (bytes == 0) bytes = 256;
+        if (hipMalloc((void**)&base, bytes) != hipSuccess) {
+            base = nullptr;
+            return PANO_ERR_NOMEM;
+        }
+        size = bytes;
+        used = 0;
+        return PANO_OK;
+    }
+    void* take(size_t bytes)
+    {
+        size_t a = align(bytes);
+        if (used + a > size) return nullptr;
+        void* p = base + used;
+        used += a;
+        return p;
+    }
+};
+
+// Sizes the arena in a first pass, hands out pointers in a second.
+struct Planner {
+    size_t total = 0;
+    void add(size_t bytes) { total += Arena::align(bytes); }
+};
+
+// A small ring of pinned host slots for per-frame uploads (gain LUTs); a slot
+// is reused only after the copy that read it has finished.
+struct PinnedRing {
+    PinnedRing() = default;
+    PinnedRing(const PinnedRing&) = delete;
+    PinnedRing& operator=(const PinnedRing&) = delete;
+    static constexpr int kSlots = 4;
+    uint8_t* host = nullptr;
+    size_t slot_bytes = 0;
+    hipEvent_t ev[kSlots] = {};
+    bool used[kSlots] = {};
+    int next = 0;
+    ~PinnedRing() { release(); }
+    void release()
+    {
+        if (host) (void)hipHostFree(host);
+        host = nullptr;
+        for (int i = 0; i < kSlots; i++) {
+            if (ev[i]) (void)hipEventDestroy(ev[i]);
+            ev[i] = nullptr;
+            used[i] = false;
+        }
+    }
+    int init(size_t bytes)
+    {
+        release();
+        slot_bytes = Arena::align(bytes ? bytes : 1);
+        if (hipHostMalloc((void**)&host, slot_bytes * kSlots, hipHostMallocDefault) != hipSuccess) {
+            host = nullptr;
+            return PANO_ERR_NOMEM;
+        }
+        for (int i = 0; i < kSlots; i++)
+            if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) return PANO_ERR_HIP;
+        return PANO_OK;
+    }
+    int upload(const void* src, size_t bytes, void* dst, hipStream_t s)
+    {
+        int k = next;
+        next = (next + 1) % kSlots;
+        if (used[k]) HIP_TRY(hipEventSynchronize(ev[k]));
+        uint8_t* h = host + slot_bytes * k;
+        memcpy(h, src, bytes);
+        HIP_TRY(hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipEventRecord(ev[k], s));
+        used[k] = true;
+        return PANO_OK;
+    }
+};
+
+template <class T>
+T* upload_vec(Arena& a, const std::vector<T>& v, int& err)
+{
+    if (v.empty()) return nullptr;
+    T* d = (T*)a.take(sizeof(T) * v.size());
+    if (!d) { err = PANO_ERR_NOMEM; return nullptr; }
+    if (hipMemcpy(d, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice) != hipSuccess)
+        err = PANO_ERR_HIP;
+    return d;
+}
+
+// Device residency of one prepared rig (blender or render).
+struct DeviceRig {
+    Arena arena;
+    Prepared prep;
+    int kind = kPixU8;                      // pixel kind of the internal planes
+    std::vector<Plane> planes;              // n * (L + 1), level 0 internal storage
+    std::vector<Plane> planes_user;         // the same with level 0 = caller images
+    Plane* d_planes = nullptr;
+    Plane* d_planes_user = nullptr;
+    CanvasLevel* d_canvas = nullptr;
+    RenderDesc* d_rd = nullptr;
+    std::vector<DownJob*> d_down;
+    std::vector<int> n_down;
+    std::vector<TileJob*> d_blend;
+    std::vector<int> n_blend;
+    int* d_camlist = nullptr;
+    TileJob* d_fin = nullptr;
+    RemapJob* d_remap = nullptr;
+    RemapCam* d_cams = nullptr;
+    uint8_t* d_label = nullptr;
+    uint8_t* d_masks = nullptr;             // full-canvas masks (caller-image level 0)
+    uint8_t* d_luts = nullptr;
+    size_t model_bytes[8] = {};
+
+    int levels() const { return prep.levels; }
+
+    // level0_internal: allocate level-0 planes (fused path / begin-tile-end);
+    // user_level0: describe level 0 as caller images (blend()).
+    // maps: host x/y planes for the fused path (nullptr otherwise); masks: host.
+    int build(bool level0_internal, bool keep_masks, const float* const* xmaps,
+              const float* const* ymaps, const uint8_t* const* masks, const size_t* mstep,
+              const int* src_w, const int* src_h)
+    {
+        const int n = prep.n, L = prep.levels;
+        const size_t P = (size_t)prep.rows * prep.cols;
+        const size_t px = kind == kPixU8 ? 4 : 8;
+        Planner pl;
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j <= L; j++) {
+                const Geom& g = prep.geom[(size_t)i * (L + 1) + j];
+                if (j > 0 || level0_internal) pl.add(g.pixels() * px);
+                if (j > 0) pl.add(g.pixels() * 2);
+            }
+        for (int j = 1; j <= L; j++) {
+            pl.add((size_t)prep.lw[j] * prep.lh[j] * 16);
+            if (!prep.full) pl.add((size_t)prep.lw[j] * prep.lh[j] * 2);
+        }
+        pl.add(sizeof(Plane) * n * (L + 1));
+        pl.add(sizeof(Plane) * n * (L + 1));
+        pl.add(sizeof(CanvasLevel) * (L + 1));
+        pl.add(sizeof(RenderDesc));
+        for (int j = 0; j < L; j++) pl.add(sizeof(DownJob) * prep.down[j].size());
+        for (int j = 1; j <= L; j++) pl.add(sizeof(TileJob) * prep.blend[j].size());
+        pl.add(sizeof(int) * prep.camlist.size());
+        pl.add(sizeof(TileJob) * prep.fin.size());
+        pl.add(P);  // label
+        if (keep_masks) pl.add(P * n);
+        if (xmaps) {
+            pl.add(sizeof(RemapJob) * prep.remap.size());
+            pl.add(sizeof(RemapCam) * n);
+            for (int i = 0; i < n; i++) {
+                const Geom& g = prep.geom[(size_t)i * (L + 1)];
+                pl.add(g.pixels() * 8);
+                pl.add(g.pixels());
+            }
+            pl.add((size_t)n * 768);
+        }
+        pl.add(64 << 10);  // slack for per-piece alignment
+        int err = arena.reserve(pl.total);
+        if (err) return err;
+
+        planes.assign((size_t)n * (L + 1), Plane());
+        for (int i = 0; i < n; i++)
+            for (int j = 0; j <= L; j++) {
+                const Geom& g = prep.geom[(size_t)i * (L + 1) + j];
+                Plane& p = planes[(size_t)i * (L + 1) + j];
+                p.x0 = g.x0; p.y0 = g.y0; p.bw = g.bw; p.bh = g.bh;
+                p.cw = g.cw; p.ch = g.ch; p.pitch = g.pitch; p.src = -1;
+                if (j > 0 || level0_internal) {
+                    p.g = arena.take(g.pixels() * px);
+                    if (!p.g && g.pixels()) return PANO_ERR_NOMEM;
+                    if (p.g) HIP_TRY(hipMemset(p.g, 0, g.pixels() * px));
+                }
+                if (j > 0) {
+                    p.w = (uint16_t*)arena.take(g.pixels() * 2);
+                    const std::vector<uint16_t>& w = prep.weight[(size_t)i * (L + 1) + j];
+                    if (!w.empty()) HIP_TRY(hipMemcpy(p.w, w.data(), w.size() * 2, hipMemcpyHostToDevice));
+                }
+            }
+        std::vector<CanvasLevel> canvas(L + 1);
+        for (int j = 1; j <= L; j++) {
+            CanvasLevel& c = canvas[j];
+            c.cw = prep.lw[j];
+            c.ch = prep.lh[j];
+            c.pitch = prep.lw[j];
+            c.c = (int4*)arena.take((size_t)c.cw * c.ch * 16);
+            if (!prep.full) {
+                c.wsum = (uint16_t*)arena.take((size_t)c.cw * c.ch * 2);
+                HIP_TRY(hipMemcpy(c.wsum, prep.wsum[j].data(), prep.wsum[j].size() * 2,
+                                  hipMemcpyHostToDevice));
+            }
+        }
+        canvas[0].cw = prep.cols;
+        canvas[0].ch = prep.rows;
+        d_label = (uint8_t*)arena.take(P);
+        HIP_TRY(hipMemcpy(d_label, prep.label.data(), P, hipMemcpyHostToDevice));
+        if (keep_masks) {
+            d_masks = (uint8_t*)arena.take(P * n);
+            for (int i = 0; i < n; i++)
+                HIP_TRY(hipMemcpy2D(d_masks + P * i, prep.cols, masks[i], mstep[i], prep.cols,
+                                    prep.rows, hipMemcpyHostToDevice));
+        }
+        planes_user = planes;
+        for (int i = 0; i < n; i++) {
+            Plane& p = planes_user[(size_t)i * (L + 1)];
+            p.g = nullptr;
+            p.src = i;
+            if (d_masks) {
+                p.mask = d_masks + P * i;
+                p.mask_pitch = prep.cols;
+            }
+            if (d_masks) {
+                planes[(size_t)i * (L + 1)].mask = d_masks + P * i;
+                planes[(size_t)i * (L + 1)].mask_pitch = prep.cols;
+            }
+        }
+        d_planes = upload_vec(arena, planes, err);
+        d_planes_user = upload_vec(arena, planes_user, err);
+        d_canvas = upload_vec(arena, canvas, err);
+        RenderDesc rd{};
+        rd.label = d_label;
+        rd.label_pitch = prep.cols;
+        rd.full = prep.full;
+        rd.levels = L;
+        rd.plane0 = 0;
+        rd.canvas0 = 0;
+        rd.out = 0;
+        d_rd = upload_vec(arena, std::vector<RenderDesc>{rd}, err);
+        d_down.assign(L, nullptr);
+        n_down.assign(L, 0);
+        for (int j = 0; j < L; j++) {
+            d_down[j] = upload_vec(arena, prep.down[j], err);
+            n_down[j] = (int)prep.down[j].size();
+        }
+        d_blend.assign(L + 1, nullptr);
+        n_blend.assign(L + 1, 0);
+        for (int j = 1; j <= L; j++) {
+            d_blend[j] = upload_vec(arena, prep.blend[j], err);
+            n_blend[j] = (int)prep.blend[j].size();
+        }
+        d_camlist = upload_vec(arena, prep.camlist, err);
+        d_fin = upload_vec(arena, prep.fin, err);
+        if (err) return err;
+        if (xmaps) {
+            d_remap = upload_vec(arena, prep.remap, err);
+            std::vector<RemapCam> cams(n);
+            for (int i = 0; i < n; i++) {
+                const Geom& g = prep.geom[(size_t)i * (L + 1)];
+                RemapCam& c = cams[i];
+                c.sw = src_w[i];
+                c.sh = src_h[i];
+                c.src = i;
+                c.lut = i * 768;
+                c.plane = i * (L + 1);
+                float2* dm = (float2*)arena.take(g.pixels() * 8);
+                uint8_t* dk = (uint8_t*)arena.take(g.pixels());
+                std::vector<float2> hm(g.pixels(), make_float2(-1.f, -1.f));
+                std::vector<uint8_t> hk(g.pixels(), 0);
+                for (int ly = 0; ly < g.bh; ly++)
+                    for (int lx = 0; lx < g.bw; lx++) {
+                        int X = (g.x0 + lx) % g.cw, Y = g.y0 + ly;
+                        size_t o = (size_t)Y * prep.cols + X;
+                        hm[(size_t)ly * g.pitch + lx] = make_float2(xmaps[i][o], ymaps[i][o]);
+                        hk[(size_t)ly * g.pitch + lx] = masks[i][(size_t)Y * mstep[i] + X];
+                    }
+                if (g.pixels()) {
+                    HIP_TRY(hipMemcpy(dm, hm.data(), g.pixels() * 8, hipMemcpyHostToDevice));
+                    HIP_TRY(hipMemcpy(dk, hk.data(), g.pixels(), hipMemcpyHostToDevice));
+                }
+                c.map = dm;
+                c.mask = dk;
+            }
+            d_cams = upload_vec(arena, cams, err);
+            d_luts = (uint8_t*)arena.take((size_t)n * 768);
+            if (err) return err;
+        }
+        compute_model_bytes(xmaps != nullptr, src_w, src_h);
+        return err;
+    }
+
+    // Algorithmic bytes per stage: every input read once, every output written
+    // once.  stage 0 remap, 1 down (all levels), 2 blend (levels >= 1), 3 final.
+    void compute_model_bytes(bool fused, const int* src_w, const int* src_h)
+    {
+        const int n = prep.n, L = prep.levels;
+        const size_t px = kind == kPixU8 ? 4 : 8;
+        size_t remap = 0, down = 0, blend = 0, fin = 0;
+        for (int i = 0; i < n; i++) {
+            const Geom& g0 = prep.geom[(size_t)i * (L + 1)];
+            size_t a0 = (size_t)g0.bw * g0.bh;
+            if (fused) remap += a0 * (8 + 1 + px) + (size_t)src_w[i] * src_h[i] * 4;
+            for (int j = 0; j < L; j++) {
+                const Geom& gs = prep.geom[(size_t)i * (L + 1) + j];
+                const Geom& gd = prep.geom[(size_t)i * (L + 1) + j + 1];
+                down += (size_t)gs.bw * gs.bh * px + (size_t)gd.bw * gd.bh * px;
+            }
+        }
+        for (int j = 1; j <= L; j++) {
+            size_t cp = (size_t)prep.lw[j] * prep.lh[j];
+            blend += cp * 16;                          // C_j write
+            if (j < L) blend += (size_t)prep.lw[j + 1] * prep.lh[j + 1] * 16;  // C_{j+1} read
+            if (!prep.full) blend += cp * 2;
+            for (const TileJob& t : prep.blend[j])
+                blend += (size_t)t.nx * t.ny * t.cnt * (px + 2) +
+                         (j < L ? (size_t)t.nx * t.ny * t.cnt * px / 4 : 0);
+        }
+        size_t P = (size_t)prep.rows * prep.cols;
+        fin = P * (1 + px + px / 4 + 4) + (L > 0 ? (size_t)prep.lw[1] * prep.lh[1] * 16 : 0);
+        model_bytes[0] = remap;
+        model_bytes[1] = down;
+        model_bytes[2] = blend;
+        model_bytes[3] = fin;
+        model_bytes[4] = remap + down + blend + fin;
+    }
+
+    int run_pyramid(int user_kind, const SrcTable& st, const OutTable& ot, hipStream_t s)
+    {
+        const int L = prep.levels;
+        const Plane* pl = user_kind >= 0 ? d_planes_user : d_planes;
+        for (int j = 0; j < L; j++)
+            HIP_TRY(launch_down(kind, j == 0 ? user_kind : -1, d_down[j], n_down[j], pl, st, s));
+        for (int j = L; j >= 1; j--)
+            HIP_TRY(launch_blend(kind, d_blend[j], n_blend[j], d_camlist, pl, d_canvas, d_rd, j, s));
+        HIP_TRY(launch_final(kind, user_kind, d_fin, (int)prep.fin.size(), pl, d_canvas, d_rd, st,
+                             ot, s));
+        return PANO_OK;
+    }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ objects --
+
+struct pano_blender {
+    std::unique_ptr<DeviceRig> rig;
+    bool ok = false;
+    bool tiling = false;
+    std::vector<uint8_t> tiled;
+};
+
+struct pano_render {
+    std::unique_ptr<DeviceRig> rig;
+    bool ok = false;
+    std::vector<int> src_w, src_h;
+    int dst_w = 0, dst_h = 0;
+    PinnedRing ring;
+};
+
+extern "C" {
+
+const char* pano_status_string(int status)
+{
+    switch (status) {
+    case PANO_OK: return "ok";
+    case PANO_ERR_ARG: return "invalid argument";
+    case PANO_ERR_STATE: return "not prepared";
+    case PANO_ERR_HIP: return "HIP runtime error";
+    case PANO_ERR_NOMEM: return "out of memory";
+    case PANO_ERR_UNSUPPORTED: return "unsupported";
+    case PANO_ERR_BUDGET: return "device memory budget exceeded";
+    default: return "unknown status";
+    }
+}
+
+int pano_version(void) { return 100; }
+
+// ------------------------------------------------------------------ blender --
+
+int pano_blender_create(pano_blender** out)
+{
+    if (!out) return PANO_ERR_ARG;
+    *out = new (std::nothrow) pano_blender();
+    return *out ? PANO_OK : PANO_ERR_NOMEM;
+}
+
+void pano_blender_destroy(pano_blender* b) { delete b; }
+
+int pano_blender_prepare(pano_blender* b, int num, int rows, int cols, const uint8_t* const* masks,
+                         const size_t* mask_steps, int max_levels, int min_length)
+{
+    if (!b) return PANO_ERR_ARG;
+    b->ok = false;
+    b->rig.reset(new (std::nothrow) DeviceRig());
+    if (!b->rig) return PANO_ERR_NOMEM;
+    b->rig->kind = kPixS16;
+    int err = prepare_rig(num, rows, cols, masks, mask_steps, nullptr, max_levels, min_length,
+                          true, b->rig->prep);
+    if (err) return err;
+    err = b->rig->build(true, true, nullptr, nullptr, masks, mask_steps, nullptr, nullptr);
+    if (err) return err;
+    b->ok = true;
+    return PANO_OK;
+}
+
+int pano_blender_prepare_device(pano_blender* b, int num, int rows, int cols,
+                                const uint8_t* const* masks, const size_t* mask_steps,
+                                int max_levels, int min_length)
+{
+    if (!b || num <= 0 || rows <= 0 || cols <= 0 || !masks || !mask_steps) return PANO_ERR_ARG;
+    std::vector<std::vector<uint8_t>> host(num, std::vector<uint8_t>((size_t)rows * cols));
+    std::vector<const uint8_t*> ptrs(num);
+    std::vector<size_t> steps(num, (size_t)cols);
+    for (int i = 0; i < num; i++) {
+        if (!masks[i]) return PANO_ERR_ARG;
+        HIP_TRY(hipMemcpy2D(host[i].data(), cols, masks[i], mask_steps[i], cols, rows,
+                            hipMemcpyDeviceToHost));
+        ptrs[i] = host[i].data();
+    }
+    return pano_blender_prepare(b, num, rows, cols, ptrs.data(), steps.data(), max_levels,
+                                min_length);
+}
+
+int pano_blender_blend(pano_blender* b, int num, const void* const* images, const size_t* steps,
+                       int type, uint8_t* out, size_t out_step, hipStream_t stream)
+{
+    if (!b) return PANO_ERR_ARG;
+    if (!b->ok) return PANO_ERR_STATE;
+    DeviceRig& r = *b->rig;
+    if (num != r.prep.n || !images || !steps || !out) return PANO_ERR_ARG;
+    if (type != PANO_16SC4 && type != PANO_8UC4) return PANO_ERR_ARG;
+    if (num > kMaxSrc) return PANO_ERR_UNSUPPORTED;
+    size_t px = type == PANO_16SC4 ? 8 : 4;
+    if (out_step < (size_t)r.prep.cols * 4 || out_step % 4) return PANO_ERR_ARG;
+    SrcTable st{};
+    for (int i = 0; i < num; i++) {
+        if (!images[i] || steps[i] < (size_t)r.prep.cols * px || steps[i] % px) return PANO_ERR_ARG;
+        st.p[i] = (const uint8_t*)images[i];
+        st.step[i] = (uint32_t)steps[i];
+    }
+    OutTable ot{};
+    ot.p[0] = out;
+    ot.step[0] = (uint32_t)out_step;
+    return r.run_pyramid(type == PANO_16SC4 ? kPixS16 : kPixU8, st, ot, stream);
+}
+
+int pano_blender_begin(pano_blender* b)
+{
+    if (!b) return PANO_ERR_ARG;
+    if (!b->ok) return PANO_ERR_STATE;
+    b->tiling = true;
+    b->tiled.assign(b->rig->prep.n, 0);
+    return PANO_OK;
+}
+
+int pano_blender_tile(pano_blender* b, const void* image, size_t step, int type, int index,
+                      hipStream_t stream)
+{
+    if (!b) return PANO_ERR_ARG;
+    if (!b->ok || !b->tiling) return PANO_ERR_STATE;
+    DeviceRig& r = *b->rig;
+    if (index < 0 || index >= r.prep.n || !image) return PANO_ERR_ARG;
+    if (type != PANO_16SC4 && type != PANO_8UC4) return PANO_ERR_ARG;
+    size_t px = type == PANO_16SC4 ? 8 : 4;
+    if (step < (size_t)r.prep.cols * px || step % px) return PANO_ERR_ARG;
+    const Plane& p = r.planes[(size_t)index * (r.prep.levels + 1)];
+    HIP_TRY(launch_import(type == PANO_16SC4 ? kPixS16 : kPixU8, image, step, p.mask,
+                          p.mask_pitch, p.g, p.pitch, r.prep.cols, r.prep.rows, stream));
+    b->tiled[index] = 1;
+    return PANO_OK;
+}
+
+int pano_blender_end(pano_blender* b, uint8_t* out, size_t out_step, hipStream_t stream)
+{
+    if (!b) return PANO_ERR_ARG;
+    if (!b->ok || !b->tiling) return PANO_ERR_STATE;
+    DeviceRig& r = *b->rig;
+    if (!out || out_step < (size_t)r.prep.cols * 4 || out_step % 4) return PANO_ERR_ARG;
+    for (int i = 0; i < r.prep.n; i++)
+        if (!b->tiled[i]) {
+            // a camera that was never tiled contributes a black image
+            const Plane& p = r.planes[(size_t)i * (r.prep.levels + 1)];
+            HIP_TRY(launch_zero(p.g, (size_t)p.pitch * p.bh * 8, stream));
+        }
+    SrcTable st{};
+    OutTable ot{};
+    ot.p[0] = out;
+    ot.step[0] = (uint32_t)out_step;
+    int err = r.run_pyramid(-1, st, ot, stream);
+    b->tiling = false;
+    return err;
+}
+
+int pano_blender_unique_masks(const pano_blender* b, uint8_t* const* out, const size_t* steps)
+{
+    if (!b || !out || !steps) return PANO_ERR_ARG;
+    if (!b->ok) return PANO_ERR_STATE;
+    const Prepared& p = b->rig->prep;
+    const size_t P = (size_t)p.rows * p.cols;
+    for (int i = 0; i < p.n; i++) {
+        if (!out[i] || steps[i] < (size_t)p.cols) return PANO_ERR_ARG;
+        for (int y = 0; y < p.rows; y++)
+            memcpy(out[i] + (size_t)y * steps[i], p.unique.data() + P * i + (size_t)y * p.cols, p.cols);
+    }
+    return PANO_OK;
+}
+
+int pano_blender_num_levels(const pano_blender* b) { return b && b->ok ? b->rig->prep.levels : -1; }
+
+int64_t pano_blender_mem_bytes(const pano_blender* b)
+{
+    return b && b->ok ? (int64_t)b->rig->arena.used : 0;
+}
+
+int64_t pano_estimate_mem_bytes(int width, int height, int num)
+{
+    if (width <= 0 || height <= 0 || num <= 0) return 0;
+    // full-canvas 16-bit pyramids (x 4/3) + weights (levels >= 1) + level-0 copies,
+    // collapsed result pyramid (32-bit x 4, levels >= 1), masks and label
+    double P = (double)width * height;
+    double per_cam = P * 8 * (1.0 / 3) + P * 2 * (1.0 / 3) + P * 8 + P;
+    double canvas = P * 16 * (1.0 / 3) + P * 2 * (1.0 / 3) + P;
+    return (int64_t)(num * per_cam + canvas);
+}
+
+// ------------------------------------------------------------ free functions --
+
+int pano_reproject_to16s(const uint8_t* src, size_t src_step, int src_w, int src_h,
+                         const float* xmap, const float* ymap, size_t map_step, int16_t* dst,
+                         size_t dst_step, int dst_w, int dst_h, hipStream_t stream)
+{
+    if (!src || !xmap || !ymap || !dst || src_w <= 0 || src_h <= 0 || dst_w <= 0 || dst_h <= 0)
+        return PANO_ERR_ARG;
+    if (src_step < (size_t)src_w * 4 || src_step % 4 || map_step < (size_t)dst_w * 4 ||
+        map_step % 4 || dst_step < (size_t)dst_w * 8 || dst_step % 8 || src_step > 0xffffffffu)
+        return PANO_ERR_ARG;
+    HIP_TRY(launch_reproject16s(src, src_step, src_w, src_h, xmap, ymap, map_step, dst, dst_step,
+                                dst_w, dst_h, stream));
+    return PANO_OK;
+}
+
+int pano_transform_lut3(const uint8_t* src, size_t src_step, uint8_t* dst, size_t dst_step, int w,
+                        int h, const uint8_t* luts, hipStream_t stream)
+{
+    if (!src || !dst || !luts || w <= 0 || h <= 0) return PANO_ERR_ARG;
+    if (src_step < (size_t)w * 4 || dst_step < (size_t)w * 4 || src_step % 4 || dst_step % 4)
+        return PANO_ERR_ARG;
+    // the LUT travels with the work on the stream: a device copy owned by the
+    // stream order (freed asynchronously after the kernel)
+    uint8_t* d = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&d, 768, stream));
+    HIP_TRY(hipMemcpyAsync(d, luts, 768, hipMemcpyHostToDevice, stream));
+    hipError_t e = launch_transform(src, src_step, dst, dst_step, w, h, d, stream);
+    hipError_t f = hipFreeAsync(d, stream);
+    return (e != hipSuccess || f != hipSuccess) ? PANO_ERR_HIP : PANO_OK;
+}
+
+int pano_get_lut(uint8_t* lut, double k)
+{
+    if (!lut || !(k > 0)) return PANO_ERR_ARG;
+    for (int i = 0; i < 256; i++) {
+        double v = k * i;
+        long r = (v > -2147483648.5 && v < 2147483647.5) ? lrint(v) : -2147483648L;
+        lut[i] = (uint8_t)(r < 0 ? 0 : (r > 255 ? 255 : r));
+    }
+    return PANO_OK;
+}
+
+int pano_exposure_luts(int num, const double* exposures, const double* red_ratios,
+                       const double* blue_ratios, uint8_t* luts)
+{
+    if (num <= 0 || !exposures || !red_ratios || !blue_ratios || !luts) return PANO_ERR_ARG;
+    for (int i = 0; i < num; i++) {
+        int e = pano_get_lut(luts + (size_t)i * 768, exposures[i] * blue_ratios[i]);
+        if (!e) e = pano_get_lut(luts + (size_t)i * 768 + 256, exposures[i]);
+        if (!e) e = pano_get_lut(luts + (size_t)i * 768 + 512, exposures[i] * red_ratios[i]);
+        if (e) return e;
+    }
+    return PANO_OK;
+}
+
+int pano_generate_map(const pano_photo_param* param, int src_w, int src_h, int dst_w, int dst_h,
+                      float* xmap, float* ymap, uint8_t* mask)
+{
+    if (!param) return PANO_ERR_ARG;
+    return generate_map(*param, src_w, src_h, dst_w, dst_h, xmap, ymap, mask);
+}
+
+// ------------------------------------------------------------------- render --
+
+int pano_render_create(pano_render** out)
+{
+    if (!out) return PANO_ERR_ARG;
+    *out = new (std::nothrow) pano_render();
+    return *out ? PANO_OK : PANO_ERR_NOMEM;
+}
+
+void pano_render_destroy(pano_render* r) { delete r; }
+
+int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const int* src_h,
+                             int dst_w, int dst_h, const float* const* xmaps,
+                             const float* const* ymaps, const uint8_t* const* masks,
+                             int max_levels)
+{
+    if (!r) return PANO_ERR_ARG;
+    r->ok = false;
+    if (num <= 0 || num > kMaxSrc || !src_w || !src_h || !xmaps || !ymaps || !masks)
+        return PANO_ERR_ARG;
+    if ((dst_w & 1) || (dst_h & 1) || dst_h * 2 != dst_w || dst_w <= 0) return PANO_ERR_ARG;
+    if (max_levels <= 0) return PANO_ERR_ARG;
+    const size_t P = (size_t)dst_w * dst_h;
+    std::vector<size_t> steps(num, (size_t)dst_w);
+    // pixels whose level-0 value can be non-zero: mask or a valid map point
+    std::vector<std::vector<uint8_t>> support(num, std::vector<uint8_t>(P));
+    std::vector<const uint8_t*> sp(num);
+    bool consistent = true;
+    for (int i = 0; i < num; i++) {
+        if (!xmaps[i] || !ymaps[i] || !masks[i] || src_w[i] <= 0 || src_h[i] <= 0) return PANO_ERR_ARG;
+        for (size_t p = 0; p < P; p++) {
+            double x = xmaps[i][p], y = ymaps[i][p];
+            bool v = x >= 0 && y >= 0 && x < src_w[i] && y < src_h[i];
+            support[i][p] = v;
+            if (v && !masks[i][p]) consistent = false;
+        }
+        sp[i] = support[i].data();
+    }
+    r->rig.reset(new (std::nothrow) DeviceRig());
+    if (!r->rig) return PANO_ERR_NOMEM;
+    // 8-bit planes hold every Gaussian level exactly when sampled pixels lie
+    // inside the masks (as getReprojectMapAndMask produces them); otherwise the
+    // reference's pyramid can leave [0, 255] and 16-bit planes are used.
+    r->rig->kind = consistent ? kPixU8 : kPixS16;
+    int err = prepare_rig(num, dst_h, dst_w, masks, steps.data(), sp.data(), max_levels, 2, false,
+                          r->rig->prep);
+    if (err) return err;
+    err = r->rig->build(true, false, xmaps, ymaps, masks, steps.data(), src_w, src_h);
+    if (err) return err;
+    err = r->ring.init((size_t)num * 768);
+    if (err) return err;
+    r->src_w.assign(src_w, src_w + num);
+    r->src_h.assign(src_h, src_h + num);
+    r->dst_w = dst_w;
+    r->dst_h = dst_h;
+    r->ok = true;
+    return PANO_OK;
+}
+
+int pano_render_prepare(pano_render* r, int num, const pano_photo_param* params, const int* src_w,
+                        const int* src_h, int dst_w, int dst_h, int max_levels)
+{
+    if (!r || num <= 0 || !params || !src_w || !src_h) return PANO_ERR_ARG;
+    if ((dst_w & 1) || (dst_h & 1) || dst_h * 2 != dst_w || dst_w <= 0) return PANO_ERR_ARG;
+    const size_t P = (size_t)dst_w * dst_h;
+    std::vector<std::vector<float>> xm(num, std::vector<float>(P)), ym(num, std::vector<float>(P));
+    std::vector<std::vector<uint8_t>> mk(num, std::vector<uint8_t>(P));
+    std::vector<const float*> xp(num), yp(num);
+    std::vector<const uint8_t*> kp(num);
+    for (int i = 0; i < num; i++) {
+        int e = generate_map(params[i], src_w[i], src_h[i], dst_w, dst_h, xm[i].data(),
+                             ym[i].data(), mk[i].data());
+        if (e) return e;
+        xp[i] = xm[i].data();
+        yp[i] = ym[i].data();
+        kp[i] = mk[i].data();
+    }
+    return pano_render_prepare_maps(r, num, src_w, src_h, dst_w, dst_h, xp.data(), yp.data(),
+                                    kp.data(), max_levels);
+}
+
+int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
+                       const size_t* src_steps, const uint8_t* luts, uint8_t* dst,
+                       size_t dst_step, hipStream_t stream)
+{
+    if (!r) return PANO_ERR_ARG;
+    if (!r->ok) return PANO_ERR_STATE;
+    DeviceRig& d = *r->rig;
+    if (num != d.prep.n || !srcs || !src_steps || !dst) return PANO_ERR_ARG;
+    if (dst_step < (size_t)r->dst_w * 4 || dst_step % 4) return PANO_ERR_ARG;
+    SrcTable st{};
+    for (int i = 0; i < num; i++) {
+        if (!srcs[i] || src_steps[i] < (size_t)r->src_w[i] * 4 || src_steps[i] % 4 ||
+            src_steps[i] > 0xffffffffu)
+            return PANO_ERR_ARG;
+        st.p[i] = srcs[i];
+        st.step[i] = (uint32_t)src_steps[i];
+    }
+    if (luts) {
+        int e = r->ring.upload(luts, (size_t)num * 768, d.d_luts, stream);
+        if (e) return e;
+    }
+    HIP_TRY(launch_remap(d.kind, d.d_remap, (int)d.prep.remap.size(), d.d_cams, d.d_planes,
+                         d.d_luts, luts != nullptr, st, stream));
+    OutTable ot{};
+    ot.p[0] = dst;
+    ot.step[0] = (uint32_t)dst_step;
+    return d.run_pyramid(-1, st, ot, stream);
+}
+
+int pano_render_num_images(const pano_render* r) { return r && r->ok ? r->rig->prep.n : 0; }
+int pano_render_num_levels(const pano_render* r) { return r && r->ok ? r->rig->prep.levels : -1; }
+int64_t pano_render_mem_bytes(const pano_render* r)
+{
+    return r && r->ok ? (int64_t)r->rig->arena.used : 0;
+}
+int64_t pano_render_model_bytes(const pano_render* r, int stage)
+{
+    if (!r || !r->ok || stage < 0 || stage > 4) return -1;
+    return (int64_t)r->rig->model_bytes[stage];
+}
+
+}  // extern "C"

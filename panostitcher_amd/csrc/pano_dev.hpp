@@ -1,0 +1,90 @@
+// pano_dev.hpp -- descriptors shared by the host code and the gfx950 kernels.
+//
+// Layout in HBM (per prepared rig):
+//   * per camera i and pyramid level j a Plane: the bounding box of the
+//     camera's support on the level-j canvas (wrapping horizontally), holding
+//     Gaussian pixels G_ij as 4 x u8 (B, G, R, valid) or 4 x s16, and, for
+//     j >= 1, the blend weight W_ij (u16, 0..256) with the same geometry;
+//   * per level a CanvasLevel: the collapsed result C_j (4 x s32) and, when
+//     the masks do not cover the canvas, sum_i W_ij (u16);
+//   * a level-0 owner label per canvas pixel (u8, 255 = uncovered): at level 0
+//     the reference's weights are exactly 256 on each camera's seam region, so
+//     the blended Laplacian there is the owner camera's own;
+//   * work lists of tiles for each stage.
+#pragma once
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace pano {
+
+constexpr int kMaxSrc = 128;    // source images addressable by one launch
+constexpr int kMaxLevels = 16;  // getTrueNumLevels caps at 16 (MultiBandBlend.cpp:325)
+constexpr uint8_t kNoOwner = 255;
+
+// Tile shapes (pixels).  Down: output tile at level j+1; blend/final: canvas tile.
+constexpr int kDownTW = 64, kDownTH = 8;
+constexpr int kTileW = 64, kTileH = 4;
+constexpr int kRemapTW = 64, kRemapTH = 4;
+
+enum PixKind : int { kPixU8 = 0, kPixS16 = 1 };
+
+struct Plane {
+    void* g;          // Gaussian pixels (uchar4 or short4), box-local, row pitch in pixels
+    uint16_t* w;      // blend weights (levels >= 1) or nullptr; same geometry
+    const uint8_t* mask;  // level 0 only, when the pixels come from a caller image
+    int pitch;        // pixels per row of g and w
+    int mask_pitch;   // bytes per row of mask
+    int x0, y0;       // box origin on the level canvas
+    int bw, bh;       // box size; bw == cw means the full width (wraps)
+    int cw, ch;       // level canvas size
+    int src;          // >= 0: level-0 pixels are caller image src (SrcTable index)
+    int pad;
+};
+
+struct CanvasLevel {
+    int4* c;          // collapsed result C_j, row pitch in pixels
+    uint16_t* wsum;   // sum of the cameras' weights, or nullptr when the masks cover the canvas
+    int pitch;
+    int cw, ch;
+    int pad;
+};
+
+struct RenderDesc {   // per render (rig) of a launch
+    const uint8_t* label;  // level-0 owner per canvas pixel
+    int label_pitch;
+    int full;              // masks cover the whole canvas
+    int levels;
+    int plane0;            // index of (camera 0, level 0) in the plane array
+    int canvas0;           // index of level 0 in the canvas array
+    int out;               // OutTable index of its output canvas
+};
+
+struct RemapCam {     // per camera of the fused path (resident)
+    const float2* map;     // box-local (x, y), pitch = level-0 plane pitch
+    const uint8_t* mask;   // box-local mask bytes, pitch = level-0 plane pitch
+    int sw, sh;            // source image size
+    int src;               // SrcTable index of its frame
+    int lut;               // byte offset of its 3 x 256 LUT in the LUT buffer, -1 = none
+    int plane;             // its level-0 plane
+    int pad;
+};
+
+struct RemapJob { int cam, lx, ly, nx, ny, pad0, pad1, pad2; };   // box-local tile, level 0
+struct DownJob { int src, dst, X, Y, nx, ny, pad0, pad1; };       // canvas tile at level j+1
+struct TileJob { int X, Y, nx, ny, r, off, cnt, flags; };         // canvas tile at level j
+
+enum TileFlags : int { kTileTop = 1 };
+
+struct SrcTable {
+    const uint8_t* p[kMaxSrc];
+    uint32_t step[kMaxSrc];
+};
+
+constexpr int kMaxOut = 32;     // output canvases addressable by one launch
+struct OutTable {
+    uint8_t* p[kMaxOut];
+    uint32_t step[kMaxOut];
+};
+
+}  // namespace pano

@@ -1,0 +1,56 @@
+// pano_host.hpp -- host-side prepare of a rig: seams, pyramid geometry,
+// blend weights and work lists (everything the per-frame kernels read).
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "pano_dev.hpp"
+
+namespace pano {
+
+// Geometry of one camera plane at one level (a box of the level canvas).
+struct Geom {
+    int x0 = 0, y0 = 0, bw = 0, bh = 0;  // box origin / size on the level canvas
+    int cw = 0, ch = 0;                  // level canvas size
+    int pitch = 0;                       // pixels per row
+    size_t pixels() const { return (size_t)pitch * bh; }
+    bool empty() const { return bw <= 0 || bh <= 0; }
+};
+
+struct Prepared {
+    int n = 0, rows = 0, cols = 0, levels = 0, full = 0;
+    std::vector<int> lw, lh;                     // level canvas sizes
+    std::vector<uint8_t> unique;                 // n * rows * cols, 0 / 255
+    std::vector<uint8_t> label;                  // rows * cols, owner or kNoOwner
+    std::vector<Geom> geom;                      // n * (levels + 1)
+    std::vector<std::vector<uint16_t>> weight;   // n * (levels + 1), box-local; level 0 empty
+    std::vector<std::vector<uint16_t>> wsum;     // levels + 1 canvas planes (pitch lw); level 0 empty
+    std::vector<std::vector<DownJob>> down;      // [j]: level j -> j+1
+    std::vector<std::vector<TileJob>> blend;     // [j], j >= 1
+    std::vector<int> camlist;                    // plane indices referenced by blend jobs
+    std::vector<TileJob> fin;                    // level-0 canvas tiles
+    std::vector<RemapJob> remap;                 // level-0 box tiles (fused path)
+    int max_tile_cams = 0;
+};
+
+// cv::distanceTransform(mask, dst, CV_DIST_L1, 3) semantics (OpenCV 3.0
+// distanceTransform_3x3 with metrics {1, 2}); dst as float distances.
+void distance_transform_l1(const uint8_t* mask, size_t step, int w, int h, float* dst);
+
+// getNonIntersectingMasks (Blend/Mask.cpp:376-397).  masks: n host planes.
+void nonintersecting_masks(int n, int rows, int cols, const uint8_t* const* masks,
+                           const size_t* steps, uint8_t* out);
+
+int true_num_levels(int width, int height, int max_levels, int min_length);
+
+// Full prepare.  support0 (optional, n planes rows x cols): pixels whose level-0
+// value may be non-zero besides the mask (e.g. where a map is valid); with
+// full_boxes every plane spans the whole canvas (caller images of any content).
+// Returns PANO_OK or a negative status.
+int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps,
+                const uint8_t* const* support0, int max_levels, int min_length, bool full_boxes,
+                Prepared& out);
+
+}  // namespace pano

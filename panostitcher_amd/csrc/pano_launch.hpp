@@ -1,0 +1,41 @@
+// pano_launch.hpp -- host-side launchers of the gfx950 kernels (kernels.hip).
+// Each returns hipSuccess or the launch error; no launch allocates memory.
+#pragma once
+#include "pano_dev.hpp"
+
+namespace pano {
+
+hipError_t launch_remap(int kind, const RemapJob* jobs, int njobs, const RemapCam* cams,
+                        const Plane* planes, const uint8_t* luts, bool use_lut,
+                        const SrcTable& src, hipStream_t s);
+
+// user_kind < 0: level j planes are internal; else level 0 is caller images of
+// that kind (then kind must be kPixS16).
+hipError_t launch_down(int kind, int user_kind, const DownJob* jobs, int njobs,
+                       const Plane* planes, const SrcTable& src, hipStream_t s);
+
+hipError_t launch_blend(int kind, const TileJob* jobs, int njobs, const int* camlist,
+                        const Plane* planes, const CanvasLevel* canvas, const RenderDesc* rd,
+                        int level, hipStream_t s);
+
+hipError_t launch_final(int kind, int user_kind, const TileJob* jobs, int njobs,
+                        const Plane* planes, const CanvasLevel* canvas, const RenderDesc* rd,
+                        const SrcTable& src, const OutTable& outs, hipStream_t s);
+
+hipError_t launch_import(int user_kind, const void* img, size_t step, const uint8_t* mask,
+                         int mask_pitch, void* dst, int pitch, int w, int h, hipStream_t s);
+
+hipError_t launch_reproject16s(const uint8_t* src, size_t src_step, int sw, int sh,
+                               const float* xmap, const float* ymap, size_t map_step,
+                               int16_t* dst, size_t dst_step, int dw, int dh, hipStream_t s);
+
+hipError_t launch_transform(const uint8_t* src, size_t src_step, uint8_t* dst, size_t dst_step,
+                            int w, int h, const uint8_t* lut_dev, hipStream_t s);
+
+// Zero-fill with a kernel (stays correct inside captured graphs).
+hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+
+hipError_t launch_copy_plane(const void* src, size_t src_step, void* dst, size_t dst_step,
+                             int row_bytes, int rows, hipStream_t s);
+
+}  // namespace pano
