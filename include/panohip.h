@@ -168,9 +168,19 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
 int pano_render_num_images(const pano_render* r);
 int pano_render_num_levels(const pano_render* r);
 int64_t pano_render_mem_bytes(const pano_render* r);
-/* Algorithmic HBM bytes of one render() (inputs read once, outputs written
- * once, per stage), and of each stage kernel; see DESIGN notes in the source. */
+/* Algorithmic HBM bytes of one render(): every stage input read once and every
+ * stage output written once.  stage 0 remap, 1 down, 2 blend levels >= 1,
+ * 3 level-0 blend, 4 total. */
 int64_t pano_render_model_bytes(const pano_render* r, int stage);
+
+/* Per-stage device timing of render() (no counterpart in the reference, whose
+ * timers are host-side, Tool/Timer.h:8-143): when enabled, HIP events are
+ * recorded on the render's stream between its stages.  pano_render_stage_ms
+ * waits for the recorded frames and returns the summed milliseconds of each
+ * stage -- 0 remap, 1 pyramid down (all levels), 2 blend levels >= 1, 3 level-0
+ * blend -- and the number of frames, then starts a new period. */
+int pano_render_set_timing(pano_render* r, int enable);
+int pano_render_stage_ms(pano_render* r, double* ms4, int* frames);
 
 #ifdef __cplusplus
 }

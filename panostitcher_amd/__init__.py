@@ -88,6 +88,8 @@ def lib():
             "pano_render_num_levels": ([vp], i),
             "pano_render_mem_bytes": ([vp], i64),
             "pano_render_model_bytes": ([vp, i], i64),
+            "pano_render_set_timing": ([vp, i], i),
+            "pano_render_stage_ms": ([vp, vp, C.POINTER(C.c_int)], i),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
@@ -107,7 +109,7 @@ def exported_symbols():
         "pano_exposure_luts", "pano_generate_map", "pano_render_create", "pano_render_destroy",
         "pano_render_prepare", "pano_render_prepare_maps", "pano_render_render",
         "pano_render_num_images", "pano_render_num_levels", "pano_render_mem_bytes",
-        "pano_render_model_bytes",
+        "pano_render_model_bytes", "pano_render_set_timing", "pano_render_stage_ms",
     ]
 
 
@@ -373,6 +375,18 @@ class PanoramaRender:
 
     def model_bytes(self, stage: int = 4) -> int:
         return lib().pano_render_model_bytes(self._h, stage)
+
+    STAGES = ("remap", "down", "blend", "final")
+
+    def set_timing(self, on: bool):
+        _check(lib().pano_render_set_timing(self._h, int(bool(on))), "set timing")
+
+    def stage_ms(self):
+        """Summed device milliseconds per stage since the last call, and frames."""
+        ms = np.zeros(4, np.float64)
+        nf = C.c_int()
+        _check(lib().pano_render_stage_ms(self._h, ms.ctypes.data, C.byref(nf)), "stage times")
+        return dict(zip(self.STAGES, ms.tolist())), nf.value
 
     def render(self, srcs, out, luts=None, stream=None) -> bool:
         """srcs: list of HxWx4 uint8 cuda tensors (BGRx); out: HxWx4 uint8 cuda tensor;

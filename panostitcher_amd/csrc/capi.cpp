@@ -354,16 +354,21 @@ struct DeviceRig {
         model_bytes[4] = remap + down + blend + fin;
     }
 
-    int run_pyramid(int user_kind, const SrcTable& st, const OutTable& ot, hipStream_t s)
+    // ev: optional events 2..4 recorded after the down, blend and final stages.
+    int run_pyramid(int user_kind, const SrcTable& st, const OutTable& ot, hipStream_t s,
+                    std::vector<hipEvent_t>* ev = nullptr)
     {
         const int L = prep.levels;
         const Plane* pl = user_kind >= 0 ? d_planes_user : d_planes;
         for (int j = 0; j < L; j++)
             HIP_TRY(launch_down(kind, j == 0 ? user_kind : -1, d_down[j], n_down[j], pl, st, s));
+        if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
         for (int j = L; j >= 1; j--)
             HIP_TRY(launch_blend(kind, d_blend[j], n_blend[j], d_camlist, pl, d_canvas, d_rd, j, s));
+        if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
         HIP_TRY(launch_final(kind, user_kind, d_fin, (int)prep.fin.size(), pl, d_canvas, d_rd, st,
                              ot, s));
+        if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));
         return PANO_OK;
     }
 };
@@ -379,12 +384,59 @@ struct pano_blender {
     std::vector<uint8_t> tiled;
 };
 
+// Events between the stages of each timed frame.
+struct StageTimer {
+    static constexpr int kMarks = 5;
+    bool on = false;
+    std::vector<std::vector<hipEvent_t>> pending, spare;
+    ~StageTimer()
+    {
+        for (auto* v : {&pending, &spare})
+            for (auto& set : *v)
+                for (hipEvent_t e : set) (void)hipEventDestroy(e);
+    }
+    std::vector<hipEvent_t>* begin_frame()
+    {
+        if (!on) return nullptr;
+        std::vector<hipEvent_t> set;
+        if (!spare.empty()) {
+            set = spare.back();
+            spare.pop_back();
+        } else {
+            set.resize(kMarks);
+            for (auto& e : set)
+                if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        }
+        pending.push_back(set);
+        return &pending.back();
+    }
+    int collect(double* ms, int* frames)
+    {
+        for (int k = 0; k < kMarks - 1; k++) ms[k] = 0;
+        int nf = 0;
+        for (auto& set : pending) {
+            HIP_TRY(hipEventSynchronize(set[kMarks - 1]));
+            for (int k = 0; k < kMarks - 1; k++) {
+                float t = 0;
+                HIP_TRY(hipEventElapsedTime(&t, set[k], set[k + 1]));
+                ms[k] += t;
+            }
+            nf++;
+            spare.push_back(set);
+        }
+        pending.clear();
+        *frames = nf;
+        return PANO_OK;
+    }
+};
+
 struct pano_render {
     std::unique_ptr<DeviceRig> rig;
     bool ok = false;
     std::vector<int> src_w, src_h;
     int dst_w = 0, dst_h = 0;
     PinnedRing ring;
+    StageTimer timer;
 };
 
 extern "C" {
@@ -718,12 +770,28 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
         int e = r->ring.upload(luts, (size_t)num * 768, d.d_luts, stream);
         if (e) return e;
     }
+    std::vector<hipEvent_t>* ev = r->timer.begin_frame();
+    if (ev) HIP_TRY(hipEventRecord((*ev)[0], stream));
     HIP_TRY(launch_remap(d.kind, d.d_remap, (int)d.prep.remap.size(), d.d_cams, d.d_planes,
                          d.d_luts, luts != nullptr, st, stream));
+    if (ev) HIP_TRY(hipEventRecord((*ev)[1], stream));
     OutTable ot{};
     ot.p[0] = dst;
     ot.step[0] = (uint32_t)dst_step;
-    return d.run_pyramid(-1, st, ot, stream);
+    return d.run_pyramid(-1, st, ot, stream, ev);
+}
+
+int pano_render_set_timing(pano_render* r, int enable)
+{
+    if (!r) return PANO_ERR_ARG;
+    r->timer.on = enable != 0;
+    return PANO_OK;
+}
+
+int pano_render_stage_ms(pano_render* r, double* ms4, int* frames)
+{
+    if (!r || !ms4 || !frames) return PANO_ERR_ARG;
+    return r->timer.collect(ms4, frames);
 }
 
 int pano_render_num_images(const pano_render* r) { return r && r->ok ? r->rig->prep.n : 0; }
