@@ -151,6 +151,8 @@ struct DeviceRig {
     uint8_t* d_label = nullptr;
     uint8_t* d_masks = nullptr;             // full-canvas masks (caller-image level 0)
     uint8_t* d_luts = nullptr;
+    int* d_fin_owners = nullptr;
+    uint32_t* d_g0comp = nullptr;           // owner's level-0 value per canvas pixel (8-bit render)
     size_t model_bytes[8] = {};
 
     int levels() const { return prep.levels; }
@@ -166,12 +168,16 @@ struct DeviceRig {
         const size_t P = (size_t)prep.rows * prep.cols;
         const size_t px = kind == kPixU8 ? 4 : 8;
         Planner pl;
+        const bool fused = prep.fused0;
+        if (fused) level0_internal = false;
         for (int i = 0; i < n; i++)
             for (int j = 0; j <= L; j++) {
                 const Geom& g = prep.geom[(size_t)i * (L + 1) + j];
                 if (j > 0 || level0_internal) pl.add(g.pixels() * px);
                 if (j > 0) pl.add(g.pixels() * 2);
             }
+        if (fused) pl.add(P * 4);
+        pl.add(sizeof(int) * prep.fin_owners.size());
         for (int j = 1; j <= L; j++) {
             pl.add((size_t)prep.lw[j] * prep.lh[j] * 16);
             if (!prep.full) pl.add((size_t)prep.lw[j] * prep.lh[j] * 2);
@@ -281,6 +287,12 @@ struct DeviceRig {
         }
         d_camlist = upload_vec(arena, prep.camlist, err);
         d_fin = upload_vec(arena, prep.fin, err);
+        d_fin_owners = upload_vec(arena, prep.fin_owners, err);
+        if (fused) {
+            d_g0comp = (uint32_t*)arena.take(P * 4);
+            if (!d_g0comp) return PANO_ERR_NOMEM;
+            HIP_TRY(hipMemset(d_g0comp, 0, P * 4));
+        }
         if (err) return err;
         if (xmaps) {
             d_remap = upload_vec(arena, prep.remap, err);
@@ -302,7 +314,9 @@ struct DeviceRig {
                         int X = (g.x0 + lx) % g.cw, Y = g.y0 + ly;
                         size_t o = (size_t)Y * prep.cols + X;
                         hm[(size_t)ly * g.pitch + lx] = make_float2(xmaps[i][o], ymaps[i][o]);
-                        hk[(size_t)ly * g.pitch + lx] = masks[i][(size_t)Y * mstep[i] + X];
+                        uint8_t f = masks[i][(size_t)Y * mstep[i] + X] ? kFlagValid : 0;
+                        if (prep.label[o] == i) f |= kFlagOwned;
+                        hk[(size_t)ly * g.pitch + lx] = f;
                     }
                 if (g.pixels()) {
                     HIP_TRY(hipMemcpy(dm, hm.data(), g.pixels() * 8, hipMemcpyHostToDevice));
@@ -320,38 +334,39 @@ struct DeviceRig {
     }
 
     // Algorithmic bytes per stage: every input read once, every output written
-    // once.  stage 0 remap, 1 down (all levels), 2 blend (levels >= 1), 3 final.
-    void compute_model_bytes(bool fused, const int* src_w, const int* src_h)
+    // once (staging halos and cache re-reads are not counted).  Stage 0 is the
+    // level-0 work (remap, and for the 8-bit path the fused level 0 -> 1 down),
+    // 1 the remaining down levels, 2 the blend of levels >= 1, 3 the level-0 blend.
+    void compute_model_bytes(bool fused_maps, const int* src_w, const int* src_h)
     {
         const int n = prep.n, L = prep.levels;
         const size_t px = kind == kPixU8 ? 4 : 8;
-        size_t remap = 0, down = 0, blend = 0, fin = 0;
-        for (int i = 0; i < n; i++) {
-            const Geom& g0 = prep.geom[(size_t)i * (L + 1)];
-            size_t a0 = (size_t)g0.bw * g0.bh;
-            if (fused) remap += a0 * (8 + 1 + px) + (size_t)src_w[i] * src_h[i] * 4;
-            for (int j = 0; j < L; j++) {
-                const Geom& gs = prep.geom[(size_t)i * (L + 1) + j];
-                const Geom& gd = prep.geom[(size_t)i * (L + 1) + j + 1];
-                down += (size_t)gs.bw * gs.bh * px + (size_t)gd.bw * gd.bh * px;
+        const size_t P = (size_t)prep.rows * prep.cols;
+        size_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        if (fused_maps)
+            for (int i = 0; i < n; i++) s0 += (size_t)src_w[i] * src_h[i] * 4;
+        for (int j = 0; j < L; j++)
+            for (const DownJob& d : prep.down[j]) {
+                size_t outp = (size_t)d.nx * d.ny, inp = 4 * outp;
+                if (j == 0 && prep.fused0) s0 += inp * (8 + 1) + outp * px;
+                else if (j == 0 && fused_maps) s0 += inp * (8 + 1 + px) + inp * px + outp * px;
+                else s1 += inp * px + outp * px;
+            }
+        if (prep.fused0) s0 += P * 4;  // owners' level-0 values
+        for (int j = 1; j <= L; j++) {
+            for (const TileJob& t : prep.blend[j]) {
+                size_t tp = (size_t)t.nx * t.ny;
+                s2 += tp * 16 + (prep.full ? 0 : tp * 2);                 // C_j, sum W
+                s2 += (size_t)t.cnt * tp * (px + 2);                       // G_j, W_j
+                if (j < L) s2 += (size_t)t.cnt * tp * px / 4 + tp * 16 / 4;  // G_j+1, C_j+1
             }
         }
-        for (int j = 1; j <= L; j++) {
-            size_t cp = (size_t)prep.lw[j] * prep.lh[j];
-            blend += cp * 16;                          // C_j write
-            if (j < L) blend += (size_t)prep.lw[j + 1] * prep.lh[j + 1] * 16;  // C_{j+1} read
-            if (!prep.full) blend += cp * 2;
-            for (const TileJob& t : prep.blend[j])
-                blend += (size_t)t.nx * t.ny * t.cnt * (px + 2) +
-                         (j < L ? (size_t)t.nx * t.ny * t.cnt * px / 4 : 0);
-        }
-        size_t P = (size_t)prep.rows * prep.cols;
-        fin = P * (1 + px + px / 4 + 4) + (L > 0 ? (size_t)prep.lw[1] * prep.lh[1] * 16 : 0);
-        model_bytes[0] = remap;
-        model_bytes[1] = down;
-        model_bytes[2] = blend;
-        model_bytes[3] = fin;
-        model_bytes[4] = remap + down + blend + fin;
+        s3 = P * (1 + px + 4) + (L > 0 ? P * px / 4 + P * 16 / 4 : 0);
+        model_bytes[0] = s0;
+        model_bytes[1] = s1;
+        model_bytes[2] = s2;
+        model_bytes[3] = s3;
+        model_bytes[4] = s0 + s1 + s2 + s3;
     }
 
     // ev: optional events 2..4 recorded after the down, blend and final stages.
@@ -360,6 +375,18 @@ struct DeviceRig {
     {
         const int L = prep.levels;
         const Plane* pl = user_kind >= 0 ? d_planes_user : d_planes;
+        if (prep.fused0) {
+            // level 0 -> 1 already ran fused with the remap
+            for (int j = 1; j < L; j++) HIP_TRY(launch_down_u8(d_down[j], n_down[j], pl, s));
+            if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
+            for (int j = L; j >= 1; j--)
+                HIP_TRY(launch_blend_u8(d_blend[j], n_blend[j], d_camlist, pl, d_canvas, d_rd, j, s));
+            if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
+            HIP_TRY(launch_final_u8(d_fin, (int)prep.fin.size(), d_fin_owners, pl, d_canvas, d_rd,
+                                    d_g0comp, prep.cols, ot, s));
+            if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));
+            return PANO_OK;
+        }
         for (int j = 0; j < L; j++)
             HIP_TRY(launch_down(kind, j == 0 ? user_kind : -1, d_down[j], n_down[j], pl, st, s));
         if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
@@ -477,7 +504,7 @@ int pano_blender_prepare(pano_blender* b, int num, int rows, int cols, const uin
     if (!b->rig) return PANO_ERR_NOMEM;
     b->rig->kind = kPixS16;
     int err = prepare_rig(num, rows, cols, masks, mask_steps, nullptr, max_levels, min_length,
-                          true, b->rig->prep);
+                          true, kPixS16, false, b->rig->prep);
     if (err) return err;
     err = b->rig->build(true, true, nullptr, nullptr, masks, mask_steps, nullptr, nullptr);
     if (err) return err;
@@ -713,7 +740,7 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
     // reference's pyramid can leave [0, 255] and 16-bit planes are used.
     r->rig->kind = consistent ? kPixU8 : kPixS16;
     int err = prepare_rig(num, dst_h, dst_w, masks, steps.data(), sp.data(), max_levels, 2, false,
-                          r->rig->prep);
+                          r->rig->kind, consistent, r->rig->prep);
     if (err) return err;
     err = r->rig->build(true, false, xmaps, ymaps, masks, steps.data(), src_w, src_h);
     if (err) return err;
@@ -772,8 +799,14 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
     }
     std::vector<hipEvent_t>* ev = r->timer.begin_frame();
     if (ev) HIP_TRY(hipEventRecord((*ev)[0], stream));
-    HIP_TRY(launch_remap(d.kind, d.d_remap, (int)d.prep.remap.size(), d.d_cams, d.d_planes,
-                         d.d_luts, luts != nullptr, st, stream));
+    if (d.prep.fused0) {
+        HIP_TRY(launch_remap_down0(d.d_down.empty() ? nullptr : d.d_down[0],
+                                   d.n_down.empty() ? 0 : d.n_down[0], d.d_cams, d.d_planes,
+                                   d.d_luts, luts != nullptr, st, d.d_g0comp, d.prep.cols, stream));
+    } else {
+        HIP_TRY(launch_remap(d.kind, d.d_remap, (int)d.prep.remap.size(), d.d_cams, d.d_planes,
+                             d.d_luts, luts != nullptr, st, stream));
+    }
     if (ev) HIP_TRY(hipEventRecord((*ev)[1], stream));
     OutTable ot{};
     ot.p[0] = dst;

@@ -173,12 +173,14 @@ int true_num_levels(int width, int height, int max_levels, int min_length)
 
 int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps,
                 const uint8_t* const* support0, int max_levels, int min_length, bool full_boxes,
-                Prepared& out)
+                int kind, bool fused0, Prepared& out)
 {
     if (n <= 0 || n > 255 || rows <= 0 || cols <= 0 || !masks || !steps) return PANO_ERR_ARG;
     for (int i = 0; i < n; i++)
         if (!masks[i] || steps[i] < (size_t)cols) return PANO_ERR_ARG;
     out = Prepared();
+    out.kind = kind;
+    out.fused0 = fused0;
     out.n = n;
     out.rows = rows;
     out.cols = cols;
@@ -271,6 +273,68 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
         }
     });
 
+    // validity (alpha != 0) per level on the boxes, MultiBandBlend.cpp:945-959:
+    // level 0 is the mask; level j+1 is non-zero where any level-j pixel of the
+    // 5x5 footprint is.  Used to drop work tiles whose outputs are all zero.
+    std::vector<std::vector<uint8_t>> valid((size_t)n * (L + 1));
+    parallel_for(n, [&](int i) {
+        const Geom& g0 = out.geom[(size_t)i * (L + 1)];
+        std::vector<uint8_t>& v0 = valid[(size_t)i * (L + 1)];
+        v0.assign(g0.pixels(), 0);
+        for (int ly = 0; ly < g0.bh; ly++)
+            for (int lx = 0; lx < g0.bw; lx++) {
+                int X = (g0.x0 + lx) % g0.cw, Y = g0.y0 + ly;
+                v0[(size_t)ly * g0.pitch + lx] = masks[i][(size_t)Y * steps[i] + X] != 0;
+            }
+        for (int j = 1; j <= L; j++) {
+            const Geom& gs = out.geom[(size_t)i * (L + 1) + j - 1];
+            const Geom& gd = out.geom[(size_t)i * (L + 1) + j];
+            const std::vector<uint8_t>& prev = valid[(size_t)i * (L + 1) + j - 1];
+            std::vector<uint8_t>& cur = valid[(size_t)i * (L + 1) + j];
+            cur.assign(gd.pixels(), 0);
+            auto at = [&](int X, int Y) -> int {
+                int ly = Y - gs.y0;
+                if ((unsigned)ly >= (unsigned)gs.bh) return 0;
+                int lx = X - gs.x0;
+                if (lx < 0) lx += gs.cw;
+                if ((unsigned)lx >= (unsigned)gs.bw) return 0;
+                return prev[(size_t)ly * gs.pitch + lx];
+            };
+            // rows: any over the 5 source rows of a horizontally dilated row
+            int r0 = std::max(0, 2 * gd.y0 - 2), r1 = std::min(gs.ch - 1, 2 * (gd.y0 + gd.bh - 1) + 2);
+            std::vector<uint8_t> hrow((size_t)std::max(r1 - r0 + 1, 0) * std::max(gd.bw, 1), 0);
+            for (int r = r0; r <= r1 && gd.bw > 0; r++)
+                for (int lx = 0; lx < gd.bw; lx++) {
+                    int X = (gd.x0 + lx) % gd.cw;
+                    uint8_t a = 0;
+                    for (int b = -2; b <= 2 && !a; b++) a = (uint8_t)at(wrap_idx(2 * X + b, gs.cw), r);
+                    hrow[(size_t)(r - r0) * gd.bw + lx] = a;
+                }
+            for (int ly = 0; ly < gd.bh; ly++) {
+                int Y = gd.y0 + ly;
+                for (int lx = 0; lx < gd.bw; lx++) {
+                    uint8_t a = 0;
+                    for (int k = -2; k <= 2 && !a; k++)
+                        a = hrow[(size_t)(reflect_idx(2 * Y + k, gs.ch) - r0) * gd.bw + lx];
+                    cur[(size_t)ly * gd.pitch + lx] = a;
+                }
+            }
+        }
+    });
+    auto tile_has_valid = [&](int i, int j, int X, int Y, int nx, int ny) {
+        const Geom& g = out.geom[(size_t)i * (L + 1) + j];
+        const std::vector<uint8_t>& v = valid[(size_t)i * (L + 1) + j];
+        for (int y = Y; y < Y + ny; y++)
+            for (int x = X; x < X + nx; x++) {
+                int ly = y - g.y0, lx = x - g.x0;
+                if (lx < 0) lx += g.cw;
+                if ((unsigned)ly < (unsigned)g.bh && (unsigned)lx < (unsigned)g.bw &&
+                    v[(size_t)ly * g.pitch + lx])
+                    return true;
+            }
+        return false;
+    };
+
     // blend weights: Gaussian pyramid of 256 * unique_i (MultiBandBlend.cpp:936-943,
     // pyrDown with FixPtCast<short, 8>, WRAP columns, REFLECT_101 rows), on the boxes.
     out.weight.assign((size_t)n * (L + 1), std::vector<uint16_t>());
@@ -342,6 +406,9 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
     }
 
     // work lists
+    const bool u8 = kind == kPixU8;
+    const int dtw = u8 ? kFTW : kDownTW, dth = u8 ? kFTH : kDownTH;
+    const int ttw = u8 ? 2 * kQTW : kTileW, tth = u8 ? 2 * kQTH : kTileH;
     out.down.assign(L, std::vector<DownJob>());
     for (int j = 0; j < L; j++) {
         for (int i = 0; i < n; i++) {
@@ -350,15 +417,17 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
             int segs[2][2] = {{gd.x0, std::min(gd.x0 + gd.bw, gd.cw)}, {0, gd.x0 + gd.bw - gd.cw}};
             for (int sgi = 0; sgi < 2; sgi++) {
                 int a = segs[sgi][0], b = segs[sgi][1];
-                for (int X = a; X < b; X += kDownTW)
-                    for (int Y = gd.y0; Y < gd.y0 + gd.bh; Y += kDownTH) {
+                for (int X = a; X < b; X += dtw)
+                    for (int Y = gd.y0; Y < gd.y0 + gd.bh; Y += dth) {
                         DownJob d{};
-                        d.src = i * (L + 1) + j;
+                        d.src = (fused0 && j == 0) ? i : i * (L + 1) + j;
                         d.dst = i * (L + 1) + j + 1;
                         d.X = X;
                         d.Y = Y;
-                        d.nx = std::min(kDownTW, b - X);
-                        d.ny = std::min(kDownTH, gd.y0 + gd.bh - Y);
+                        d.nx = std::min(dtw, b - X);
+                        d.ny = std::min(dth, gd.y0 + gd.bh - Y);
+                        // outputs whose alpha is zero stay zero: skip empty tiles
+                        if (!tile_has_valid(i, j + 1, d.X, d.Y, d.nx, d.ny)) continue;
                         out.down[j].push_back(d);
                     }
             }
@@ -368,7 +437,7 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
     out.camlist.clear();
     out.max_tile_cams = 0;
     for (int j = 1; j <= L; j++) {
-        int tw = (out.lw[j] + kTileW - 1) / kTileW, th = (out.lh[j] + kTileH - 1) / kTileH;
+        int tw = (out.lw[j] + ttw - 1) / ttw, th = (out.lh[j] + tth - 1) / tth;
         std::vector<std::vector<int>> cams((size_t)tw * th);
         for (int i = 0; i < n; i++) {
             const Geom& g = out.geom[(size_t)i * (L + 1) + j];
@@ -378,7 +447,7 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
                 for (int lx = 0; lx < g.bw; lx++)
                     if (w[(size_t)ly * g.pitch + lx]) {
                         int X = (g.x0 + lx) % g.cw, Y = g.y0 + ly;
-                        hit[(size_t)(Y / kTileH) * tw + X / kTileW] = 1;
+                        hit[(size_t)(Y / tth) * tw + X / ttw] = 1;
                     }
             for (size_t t = 0; t < hit.size(); t++)
                 if (hit[t]) cams[t].push_back(i * (L + 1) + j);
@@ -386,10 +455,10 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
         for (int ty = 0; ty < th; ty++)
             for (int tx = 0; tx < tw; tx++) {
                 TileJob t{};
-                t.X = tx * kTileW;
-                t.Y = ty * kTileH;
-                t.nx = std::min(kTileW, out.lw[j] - t.X);
-                t.ny = std::min(kTileH, out.lh[j] - t.Y);
+                t.X = tx * ttw;
+                t.Y = ty * tth;
+                t.nx = std::min(ttw, out.lw[j] - t.X);
+                t.ny = std::min(tth, out.lh[j] - t.Y);
                 t.r = 0;
                 const std::vector<int>& c = cams[(size_t)ty * tw + tx];
                 t.off = (int)out.camlist.size();
@@ -401,14 +470,28 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
             }
     }
     {
-        int tw = (cols + kTileW - 1) / kTileW, th = (rows + kTileH - 1) / kTileH;
+        int tw = (cols + ttw - 1) / ttw, th = (rows + tth - 1) / tth;
+        out.fin_owners.clear();
         for (int ty = 0; ty < th; ty++)
             for (int tx = 0; tx < tw; tx++) {
                 TileJob t{};
-                t.X = tx * kTileW;
-                t.Y = ty * kTileH;
-                t.nx = std::min(kTileW, cols - t.X);
-                t.ny = std::min(kTileH, rows - t.Y);
+                t.X = tx * ttw;
+                t.Y = ty * tth;
+                t.nx = std::min(ttw, cols - t.X);
+                t.ny = std::min(tth, rows - t.Y);
+                t.off = (int)out.fin_owners.size();
+                if (u8) {
+                    uint8_t seen[256] = {};
+                    for (int y = t.Y; y < t.Y + t.ny; y++)
+                        for (int x = t.X; x < t.X + t.nx; x++) {
+                            uint8_t o = out.label[(size_t)y * cols + x];
+                            if (o != kNoOwner && !seen[o]) {
+                                seen[o] = 1;
+                                out.fin_owners.push_back(o);
+                            }
+                        }
+                }
+                t.cnt = (int)out.fin_owners.size() - t.off;
                 out.fin.push_back(t);
             }
     }

@@ -1,0 +1,396 @@
+// kernels_u8.hip -- the fused 8-bit path of render() (gfx950).
+//
+// When every sampled pixel lies inside its camera's mask (as maps from
+// getReprojectMapAndMask guarantee, ZReproject.cpp:206-215), every Gaussian
+// level of the reference's blend is an integer in [0, 255]: level 0 is a
+// saturated bilinear sample and calcDstImage (MultiBandBlend.cpp:732-768)
+// returns floor(255 v / alpha) <= 254 for v <= 255 alpha / 256.  The planes
+// are then 4 x u8 (B, G, R, valid) and the 5x5 / 3x3 filter sums run two
+// channels per 32-bit lane (no lane exceeds 16 bits: 5x5 sums <= 65280, pyrUp
+// sums <= 16320).  The arithmetic is the reference's, bit for bit:
+//   k_remap_down0  LUT + FP64 bilinear remap of a level-0 region into LDS
+//                  (never stored in HBM), pyrDown + calcDstImage -> level 1;
+//                  the owner camera's level-0 value goes to a canvas plane
+//   k_down_u8      level j -> j+1, j >= 1
+//   k_blend_u8     level j >= 1: sum_i W_ij (G_ij - up(G_i,j+1)), normalize,
+//                  + up32(C_j+1)
+//   k_final_u8     level 0: owner's G_0 - up(G_1) + up32(C_1) -> BGRx
+#include <hip/hip_runtime.h>
+
+#include "pano_dev.hpp"
+#include "pano_kernel_common.hpp"
+#include "pano_launch.hpp"
+
+namespace pano {
+
+namespace {
+
+constexpr uint32_t kLo = 0x00ff00ffu;  // (B, R) lanes of a packed pixel; >> 8 gives (G, valid)
+
+// Down tile geometry: input region (2 kFTW + 3) x (2 kFTH + 3) at level j.
+constexpr int kIW = 2 * kFTW + 3, kIH = 2 * kFTH + 3;
+
+// Horizontal 5-tap over the staged input rows, then vertical 5-tap and
+// calcDstImage; writes the (nx, ny) output tile at canvas (X, Y) of plane D.
+__device__ __forceinline__ void down_from_lds(const uint32_t* in, uint2* hs, const DownJob& J,
+                                              const Plane& D)
+{
+    const int ih = 2 * J.ny + 3;
+    for (int e = threadIdx.x; e < kIH * kFTW; e += 256) {
+        int r = e / kFTW, x = e - r * kFTW;
+        if (r >= ih || x >= J.nx) continue;
+        const uint32_t* p = in + r * kIW + 2 * x;
+        uint32_t a = p[0], b = p[1], c = p[2], d = p[3], f = p[4];
+        uint32_t lo = (a & kLo) + (f & kLo) + 4u * ((b & kLo) + (d & kLo)) + 6u * (c & kLo);
+        uint32_t hi = ((a >> 8) & kLo) + ((f >> 8) & kLo) +
+                      4u * (((b >> 8) & kLo) + ((d >> 8) & kLo)) + 6u * ((c >> 8) & kLo);
+        hs[r * kFTW + x] = make_uint2(lo, hi);
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kFTH * kFTW; e += 256) {
+        int y = e / kFTW, x = e - y * kFTW;
+        if (y >= J.ny || x >= J.nx) continue;
+        const uint2* q = hs + (2 * y) * kFTW + x;
+        uint2 a = q[0], b = q[kFTW], c = q[2 * kFTW], d = q[3 * kFTW], f = q[4 * kFTW];
+        uint32_t lo = a.x + f.x + 4u * (b.x + d.x) + 6u * c.x;
+        uint32_t hi = a.y + f.y + 4u * (b.y + d.y) + 6u * c.y;
+        int al = (int)(hi >> 16);
+        uint32_t o = 0;
+        if (al) {
+            o = (uint32_t)div_norm_u8((int)(lo & 0xffffu), al) |
+                ((uint32_t)div_norm_u8((int)(hi & 0xffffu), al) << 8) |
+                ((uint32_t)div_norm_u8((int)(lo >> 16), al) << 16) | (1u << 24);
+        }
+        int di = box_index(D, J.X + x, J.Y + y);
+        if (di >= 0) ((uint32_t*)D.g)[di] = o;
+    }
+}
+
+__device__ __forceinline__ uint32_t load_u32(const Plane& P, int X, int Y)
+{
+    int i = box_index(P, X, Y);
+    return i < 0 ? 0u : ((const uint32_t*)P.g)[i];
+}
+
+// pyrUp of a 2x2 quad from packed 8-bit pixels staged in LDS: s points at the
+// quad's source pixel (x, y); stride is the LDS row length.  Outputs (even
+// row, even col), (even, odd), (odd, even), (odd, odd) as packed u8 lanes.
+__device__ __forceinline__ void up_quad_u8(const uint32_t* s, int stride, uint32_t out[4])
+{
+    uint32_t he_lo[3], he_hi[3], ho_lo[3], ho_hi[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint32_t* row = s + (k - 1) * stride;
+        uint32_t a = row[-1], b = row[0], d = row[1];
+        he_lo[k] = (a & kLo) + 6u * (b & kLo) + (d & kLo);
+        he_hi[k] = ((a >> 8) & kLo) + 6u * ((b >> 8) & kLo) + ((d >> 8) & kLo);
+        ho_lo[k] = 4u * ((b & kLo) + (d & kLo));
+        ho_hi[k] = 4u * (((b >> 8) & kLo) + ((d >> 8) & kLo));
+    }
+    const uint32_t R = 0x00200020u;
+    uint32_t v00l = he_lo[0] + 6u * he_lo[1] + he_lo[2], v00h = he_hi[0] + 6u * he_hi[1] + he_hi[2];
+    uint32_t v01l = ho_lo[0] + 6u * ho_lo[1] + ho_lo[2], v01h = ho_hi[0] + 6u * ho_hi[1] + ho_hi[2];
+    uint32_t v10l = 4u * (he_lo[1] + he_lo[2]), v10h = 4u * (he_hi[1] + he_hi[2]);
+    uint32_t v11l = 4u * (ho_lo[1] + ho_lo[2]), v11h = 4u * (ho_hi[1] + ho_hi[2]);
+    out[0] = (((v00l + R) >> 6) & kLo) | ((((v00h + R) >> 6) & kLo) << 8);
+    out[1] = (((v01l + R) >> 6) & kLo) | ((((v01h + R) >> 6) & kLo) << 8);
+    out[2] = (((v10l + R) >> 6) & kLo) | ((((v10h + R) >> 6) & kLo) << 8);
+    out[3] = (((v11l + R) >> 6) & kLo) | ((((v11h + R) >> 6) & kLo) << 8);
+}
+
+// pyrUp<int> of a quad from int4 pixels staged in LDS (channels 0..2).
+__device__ __forceinline__ void up_quad_i32(const int4* s, int stride, int3 out[4])
+{
+    int3 he[3], ho[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int4* row = s + (k - 1) * stride;
+        int4 a = row[-1], b = row[0], d = row[1];
+        he[k] = make_int3(a.x + 6 * b.x + d.x, a.y + 6 * b.y + d.y, a.z + 6 * b.z + d.z);
+        ho[k] = make_int3(4 * (b.x + d.x), 4 * (b.y + d.y), 4 * (b.z + d.z));
+    }
+    auto fin = [](int v) { return (v + 32) >> 6; };
+    out[0] = make_int3(fin(he[0].x + 6 * he[1].x + he[2].x), fin(he[0].y + 6 * he[1].y + he[2].y),
+                       fin(he[0].z + 6 * he[1].z + he[2].z));
+    out[1] = make_int3(fin(ho[0].x + 6 * ho[1].x + ho[2].x), fin(ho[0].y + 6 * ho[1].y + ho[2].y),
+                       fin(ho[0].z + 6 * ho[1].z + ho[2].z));
+    out[2] = make_int3(fin(4 * (he[1].x + he[2].x)), fin(4 * (he[1].y + he[2].y)),
+                       fin(4 * (he[1].z + he[2].z)));
+    out[3] = make_int3(fin(4 * (ho[1].x + ho[2].x)), fin(4 * (ho[1].y + ho[2].y)),
+                       fin(4 * (ho[1].z + ho[2].z)));
+}
+
+__device__ __forceinline__ int lane(uint32_t v, int c) { return (int)((v >> (8 * c)) & 255u); }
+
+}  // namespace
+
+// ------------------------------------------------------------ level 0 -> 1 --
+
+template <bool kLut>
+__global__ __launch_bounds__(256) void k_remap_down0(const DownJob* __restrict__ jobs,
+                                                     const RemapCam* __restrict__ cams,
+                                                     const Plane* __restrict__ planes,
+                                                     const uint8_t* __restrict__ luts, SrcTable src,
+                                                     uint32_t* __restrict__ g0comp, int g0pitch)
+{
+    __shared__ uint32_t in[kIH * kIW];
+    __shared__ uint2 hs[kIH * kFTW];
+    __shared__ uint8_t lut_s[768];
+    const DownJob J = jobs[blockIdx.x];
+    const RemapCam cam = cams[J.src];
+    const bool use_lut = kLut && cam.lut >= 0;
+    if (use_lut)
+        for (int e = threadIdx.x; e < 768; e += 256) lut_s[e] = luts[cam.lut + e];
+    const Plane P0 = planes[cam.plane];
+    const Plane D = planes[J.dst];
+    const uint8_t* sp = src.p[cam.src];
+    const uint32_t sstep = src.step[cam.src];
+    __syncthreads();
+    const int bx0 = 2 * J.X - 2, by0 = 2 * J.Y - 2;
+    const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
+    for (int e = threadIdx.x; e < kIH * kIW; e += 256) {
+        int r = e / kIW, c = e - r * kIW;
+        if (r >= ih || c >= iw) continue;
+        int vx = bx0 + c, vy = by0 + r;
+        int Xs = bwrap(vx, P0.cw), Ys = breflect(vy, P0.ch);
+        int bi = box_index(P0, Xs, Ys);
+        uint32_t v = 0;
+        if (bi >= 0) {
+            uint32_t fl = cam.mask[bi];
+            float2 m = cam.map[bi];
+            double x = m.x, y = m.y;
+            if (x >= 0 && y >= 0 && x < cam.sw && y < cam.sh)
+                v = bilinear_bgr(sp, sstep, cam.sw, cam.sh, x, y, use_lut ? lut_s : nullptr);
+            if (fl & kFlagValid) v |= 1u << 24;
+            // an owned pixel of this tile's core (not a border-resolved copy)
+            if ((fl & kFlagOwned) && r >= 2 && c >= 2 && r < 2 + 2 * J.ny && c < 2 + 2 * J.nx &&
+                vx == Xs && vy == Ys)
+                g0comp[(size_t)Ys * g0pitch + Xs] = v;
+        }
+        in[r * kIW + c] = v;
+    }
+    __syncthreads();
+    down_from_lds(in, hs, J, D);
+}
+
+// ------------------------------------------------------- level j -> j+1 --
+
+__global__ __launch_bounds__(256) void k_down_u8(const DownJob* __restrict__ jobs,
+                                                 const Plane* __restrict__ planes)
+{
+    __shared__ uint32_t in[kIH * kIW];
+    __shared__ uint2 hs[kIH * kFTW];
+    const DownJob J = jobs[blockIdx.x];
+    const Plane S = planes[J.src];
+    const Plane D = planes[J.dst];
+    const int bx0 = 2 * J.X - 2, by0 = 2 * J.Y - 2;
+    const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
+    for (int e = threadIdx.x; e < kIH * kIW; e += 256) {
+        int r = e / kIW, c = e - r * kIW;
+        if (r >= ih || c >= iw) continue;
+        in[r * kIW + c] = load_u32(S, bwrap(bx0 + c, S.cw), breflect(by0 + r, S.ch));
+    }
+    __syncthreads();
+    down_from_lds(in, hs, J, D);
+}
+
+// ------------------------------------------------------- blend, level >= 1 --
+
+constexpr int kSW = kQTW + 2, kSH = kQTH + 2;  // staged level-(j+1) region
+
+__global__ __launch_bounds__(256) void k_blend_u8(const TileJob* __restrict__ jobs,
+                                                  const int* __restrict__ camlist,
+                                                  const Plane* __restrict__ planes,
+                                                  const CanvasLevel* __restrict__ canvas,
+                                                  const RenderDesc* __restrict__ rds, int level)
+{
+    __shared__ uint32_t gs[kSH * kSW];
+    __shared__ int4 cs[kSH * kSW];
+    const TileJob J = jobs[blockIdx.x];
+    const RenderDesc R = rds[J.r];
+    const bool top = J.flags & kTileTop;
+    const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
+    const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
+    const int sx0 = (J.X >> 1) - 1, sy0 = (J.Y >> 1) - 1;
+    bool in[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) in[d] = X + (d & 1) < J.X + J.nx && Y + (d >> 1) < J.Y + J.ny;
+    int acc[4][3] = {};
+    for (int k = 0; k < J.cnt; k++) {
+        const int pi = camlist[J.off + k];
+        const Plane P = planes[pi];
+        if (!top) {
+            const Plane P1 = planes[pi + 1];
+            for (int e = threadIdx.x; e < kSH * kSW; e += 256) {
+                int r = e / kSW, c = e - r * kSW;
+                gs[e] = load_u32(P1, bwrap(sx0 + c, P1.cw), breflect(sy0 + r, P1.ch));
+            }
+            __syncthreads();
+        }
+        uint32_t u[4] = {0, 0, 0, 0};
+        if (!top) up_quad_u8(gs + (qy + 1) * kSW + qx + 1, kSW, u);
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            if (!in[d]) continue;
+            int i = box_index(P, X + (d & 1), Y + (d >> 1));
+            if (i < 0) continue;
+            int w = P.w[i];
+            if (!w) continue;
+            uint32_t g = ((const uint32_t*)P.g)[i];
+#pragma unroll
+            for (int c = 0; c < 3; c++) acc[d][c] += (lane(g, c) - lane(u[d], c)) * w;
+        }
+        if (!top) __syncthreads();
+    }
+    const CanvasLevel C = canvas[R.canvas0 + level];
+    int3 up[4] = {};
+    if (!top) {
+        const CanvasLevel C1 = canvas[R.canvas0 + level + 1];
+        for (int e = threadIdx.x; e < kSH * kSW; e += 256) {
+            int r = e / kSW, c = e - r * kSW;
+            cs[e] = C1.c[breflect(sy0 + r, C1.ch) * C1.pitch + bwrap(sx0 + c, C1.cw)];
+        }
+        __syncthreads();
+        up_quad_i32(cs + (qy + 1) * kSW + qx + 1, kSW, up);
+    }
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        if (!in[d]) continue;
+        int Xp = X + (d & 1), Yp = Y + (d >> 1);
+        int a0 = acc[d][0], a1 = acc[d][1], a2 = acc[d][2];
+        if (R.full) {
+            a0 >>= 8;
+            a1 >>= 8;
+            a2 >>= 8;
+        } else {
+            int ws = C.wsum[Yp * C.pitch + Xp];
+            if (!ws) ws = 1;
+            a0 /= ws;
+            a1 /= ws;
+            a2 /= ws;
+        }
+        C.c[Yp * C.pitch + Xp] = make_int4(up[d].x + a0, up[d].y + a1, up[d].z + a2, 0);
+    }
+}
+
+// ------------------------------------------------------------ level 0 --
+
+__global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jobs,
+                                                  const int* __restrict__ owners,
+                                                  const Plane* __restrict__ planes,
+                                                  const CanvasLevel* __restrict__ canvas,
+                                                  const RenderDesc* __restrict__ rds,
+                                                  const uint32_t* __restrict__ g0comp,
+                                                  int g0pitch, OutTable outs)
+{
+    __shared__ uint32_t gs[kMaxStage][kSH * kSW];
+    __shared__ int4 cs[kSH * kSW];
+    const TileJob J = jobs[blockIdx.x];
+    const RenderDesc R = rds[J.r];
+    const int L = R.levels;
+    const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
+    const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
+    const int sx0 = (J.X >> 1) - 1, sy0 = (J.Y >> 1) - 1;
+    bool in[4];
+    int lab[4];
+    uint32_t g0[4];
+    int lap[4][3];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        int Xp = X + (d & 1), Yp = Y + (d >> 1);
+        in[d] = Xp < J.X + J.nx && Yp < J.Y + J.ny;
+        lab[d] = in[d] ? R.label[Yp * R.label_pitch + Xp] : kNoOwner;
+        g0[d] = lab[d] != kNoOwner ? g0comp[(size_t)Yp * g0pitch + Xp] : 0u;
+#pragma unroll
+        for (int c = 0; c < 3; c++) lap[d][c] = lane(g0[d], c);
+    }
+    int3 up[4] = {};
+    if (L > 0) {
+        for (int g = 0; g < J.cnt; g += kMaxStage) {
+            const int ng = J.cnt - g < kMaxStage ? J.cnt - g : kMaxStage;
+            for (int e = threadIdx.x; e < ng * kSH * kSW; e += 256) {
+                int s = e / (kSH * kSW), k = e - s * (kSH * kSW);
+                int r = k / kSW, c = k - r * kSW;
+                const Plane P1 = planes[R.plane0 + owners[J.off + g + s] * (L + 1) + 1];
+                gs[s][k] = load_u32(P1, bwrap(sx0 + c, P1.cw), breflect(sy0 + r, P1.ch));
+            }
+            __syncthreads();
+            for (int s = 0; s < ng; s++) {
+                const int o = owners[J.off + g + s];
+                if (lab[0] != o && lab[1] != o && lab[2] != o && lab[3] != o) continue;
+                uint32_t u[4];
+                up_quad_u8(&gs[s][(qy + 1) * kSW + qx + 1], kSW, u);
+#pragma unroll
+                for (int d = 0; d < 4; d++)
+                    if (lab[d] == o)
+#pragma unroll
+                        for (int c = 0; c < 3; c++) lap[d][c] = lane(g0[d], c) - lane(u[d], c);
+            }
+            __syncthreads();
+        }
+        const CanvasLevel C1 = canvas[R.canvas0 + 1];
+        for (int e = threadIdx.x; e < kSH * kSW; e += 256) {
+            int r = e / kSW, c = e - r * kSW;
+            cs[e] = C1.c[breflect(sy0 + r, C1.ch) * C1.pitch + bwrap(sx0 + c, C1.cw)];
+        }
+        __syncthreads();
+        up_quad_i32(cs + (qy + 1) * kSW + qx + 1, kSW, up);
+    }
+    uint8_t* out = outs.p[R.out];
+    const uint32_t ostep = outs.step[R.out];
+#pragma unroll
+    for (int d = 0; d < 4; d++) {
+        if (!in[d]) continue;
+        int Xp = X + (d & 1), Yp = Y + (d >> 1);
+        uint32_t o = 0;
+        if (R.full || lab[d] != kNoOwner)
+            o = (uint32_t)sat_u8(up[d].x + lap[d][0]) | ((uint32_t)sat_u8(up[d].y + lap[d][1]) << 8) |
+                ((uint32_t)sat_u8(up[d].z + lap[d][2]) << 16);
+        ((uint32_t*)(out + (size_t)Yp * ostep))[Xp] = o;
+    }
+}
+
+// ------------------------------------------------------------- launchers --
+
+hipError_t launch_remap_down0(const DownJob* jobs, int njobs, const RemapCam* cams,
+                              const Plane* planes, const uint8_t* luts, bool use_lut,
+                              const SrcTable& src, uint32_t* g0comp, int g0pitch, hipStream_t s)
+{
+    if (njobs <= 0) return hipSuccess;
+    if (use_lut)
+        hipLaunchKernelGGL(k_remap_down0<true>, dim3(njobs), dim3(256), 0, s, jobs, cams, planes,
+                           luts, src, g0comp, g0pitch);
+    else
+        hipLaunchKernelGGL(k_remap_down0<false>, dim3(njobs), dim3(256), 0, s, jobs, cams, planes,
+                           luts, src, g0comp, g0pitch);
+    return hipGetLastError();
+}
+
+hipError_t launch_down_u8(const DownJob* jobs, int njobs, const Plane* planes, hipStream_t s)
+{
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_down_u8, dim3(njobs), dim3(256), 0, s, jobs, planes);
+    return hipGetLastError();
+}
+
+hipError_t launch_blend_u8(const TileJob* jobs, int njobs, const int* camlist, const Plane* planes,
+                           const CanvasLevel* canvas, const RenderDesc* rd, int level,
+                           hipStream_t s)
+{
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_blend_u8, dim3(njobs), dim3(256), 0, s, jobs, camlist, planes, canvas, rd,
+                       level);
+    return hipGetLastError();
+}
+
+hipError_t launch_final_u8(const TileJob* jobs, int njobs, const int* owners, const Plane* planes,
+                           const CanvasLevel* canvas, const RenderDesc* rd, const uint32_t* g0comp,
+                           int g0pitch, const OutTable& outs, hipStream_t s)
+{
+    if (njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_final_u8, dim3(njobs), dim3(256), 0, s, jobs, owners, planes, canvas, rd,
+                       g0comp, g0pitch, outs);
+    return hipGetLastError();
+}
+
+}  // namespace pano

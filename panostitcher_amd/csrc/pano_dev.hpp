@@ -27,7 +27,17 @@ constexpr int kDownTW = 64, kDownTH = 8;
 constexpr int kTileW = 64, kTileH = 4;
 constexpr int kRemapTW = 64, kRemapTH = 4;
 
+// 8-bit fused path: output tile (level j+1) of the down kernels, and quad
+// tiles of the blend kernels (kQTW x kQTH quads = 2 kQTW x 2 kQTH pixels,
+// one 2x2 quad per thread).
+constexpr int kFTW = 64, kFTH = 16;
+constexpr int kQTW = 64, kQTH = 4;
+constexpr int kMaxStage = 8;    // owner cameras staged in LDS at once (level 0)
+
 enum PixKind : int { kPixU8 = 0, kPixS16 = 1 };
+
+// Per-pixel flags of a camera's level-0 box (fused path).
+enum RemapFlags : uint8_t { kFlagValid = 1, kFlagOwned = 2 };
 
 struct Plane {
     void* g;          // Gaussian pixels (uchar4 or short4), box-local, row pitch in pixels
@@ -62,7 +72,7 @@ struct RenderDesc {   // per render (rig) of a launch
 
 struct RemapCam {     // per camera of the fused path (resident)
     const float2* map;     // box-local (x, y), pitch = level-0 plane pitch
-    const uint8_t* mask;   // box-local mask bytes, pitch = level-0 plane pitch
+    const uint8_t* mask;   // box-local flags (kFlagValid | kFlagOwned), pitch = level-0 plane pitch
     int sw, sh;            // source image size
     int src;               // SrcTable index of its frame
     int lut;               // byte offset of its 3 x 256 LUT in the LUT buffer, -1 = none
@@ -72,6 +82,7 @@ struct RemapCam {     // per camera of the fused path (resident)
 
 struct RemapJob { int cam, lx, ly, nx, ny, pad0, pad1, pad2; };   // box-local tile, level 0
 struct DownJob { int src, dst, X, Y, nx, ny, pad0, pad1; };       // canvas tile at level j+1
+                                                                  // (fused level 0: src = camera)
 struct TileJob { int X, Y, nx, ny, r, off, cnt, flags; };         // canvas tile at level j
 
 enum TileFlags : int { kTileTop = 1 };

@@ -31,8 +31,11 @@ struct Prepared {
     std::vector<std::vector<TileJob>> blend;     // [j], j >= 1
     std::vector<int> camlist;                    // plane indices referenced by blend jobs
     std::vector<TileJob> fin;                    // level-0 canvas tiles
-    std::vector<RemapJob> remap;                 // level-0 box tiles (fused path)
+    std::vector<int> fin_owners;                 // owner cameras referenced by fin jobs (8-bit path)
+    std::vector<RemapJob> remap;                 // level-0 box tiles (16-bit render path)
     int max_tile_cams = 0;
+    int kind = kPixU8;                           // tile geometry of the work lists
+    bool fused0 = false;                         // down[0] jobs carry camera indices (8-bit render)
 };
 
 // cv::distanceTransform(mask, dst, CV_DIST_L1, 3) semantics (OpenCV 3.0
@@ -49,8 +52,11 @@ int true_num_levels(int width, int height, int max_levels, int min_length);
 // value may be non-zero besides the mask (e.g. where a map is valid); with
 // full_boxes every plane spans the whole canvas (caller images of any content).
 // Returns PANO_OK or a negative status.
+// kind selects the work-list geometry (kPixU8: fused 8-bit kernels, quad
+// tiles; kPixS16: generic kernels); fused0: the 8-bit render path, whose
+// level 0 -> 1 jobs address cameras instead of planes.
 int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps,
                 const uint8_t* const* support0, int max_levels, int min_length, bool full_boxes,
-                Prepared& out);
+                int kind, bool fused0, Prepared& out);
 
 }  // namespace pano

@@ -1,0 +1,168 @@
+// pano_kernel_common.hpp -- device helpers shared by the gfx950 kernels:
+// OpenCV border rules, box addressing, the pyrUp tap pattern, the FP64
+// bilinear sample and the exact calcDstImage division for 8-bit planes.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pano_dev.hpp"
+
+namespace pano {
+
+// ---------------------------------------------------------------- helpers --
+
+// cv::borderInterpolate for WRAP (columns) and REFLECT_101 (rows).
+__device__ __forceinline__ int bwrap(int p, int n)
+{
+    if ((unsigned)p < (unsigned)n) return p;
+    p %= n;
+    return p < 0 ? p + n : p;
+}
+
+__device__ __forceinline__ int breflect(int p, int n)
+{
+    if ((unsigned)p < (unsigned)n) return p;
+    if (n == 1) return 0;
+    do {
+        p = p < 0 ? -p : 2 * n - 2 - p;
+    } while ((unsigned)p >= (unsigned)n);
+    return p;
+}
+
+// Canvas pixel (X, Y), both inside the level canvas -> index into the box, or -1.
+__device__ __forceinline__ int box_index(const Plane& P, int X, int Y)
+{
+    int ly = Y - P.y0;
+    if ((unsigned)ly >= (unsigned)P.bh) return -1;
+    int lx = X - P.x0;
+    if (lx < 0) lx += P.cw;
+    if ((unsigned)lx >= (unsigned)P.bw) return -1;
+    return ly * P.pitch + lx;
+}
+
+__device__ __forceinline__ int4 unpack_u8(uint32_t v)
+{
+    return make_int4((int)(v & 255u), (int)((v >> 8) & 255u), (int)((v >> 16) & 255u),
+                     (int)(v >> 24));
+}
+
+__device__ __forceinline__ int sat_s16(int v) { return v < -32768 ? -32768 : (v > 32767 ? 32767 : v); }
+__device__ __forceinline__ int sat_u8(int v) { return v < 0 ? 0 : (v > 255 ? 255 : v); }
+
+template <int K>
+__device__ __forceinline__ int4 load_at(const void* g, int i)
+{
+    if (K == kPixU8) return unpack_u8(((const uint32_t*)g)[i]);
+    short4 s = ((const short4*)g)[i];
+    return make_int4(s.x, s.y, s.z, s.w);
+}
+
+// Gaussian pixel of a plane at canvas (X, Y); zero outside its box.
+template <int K>
+__device__ __forceinline__ int4 load_plane(const Plane& P, int X, int Y)
+{
+    int i = box_index(P, X, Y);
+    if (i < 0) return make_int4(0, 0, 0, 0);
+    return load_at<K>(P.g, i);
+}
+
+// Pixel of a caller image (level 0, full canvas).
+template <int K>
+__device__ __forceinline__ int4 load_user(const SrcTable& src, int s, int X, int Y)
+{
+    const uint8_t* row = src.p[s] + (size_t)Y * src.step[s];
+    if (K == kPixU8) return unpack_u8(((const uint32_t*)row)[X]);
+    short4 v = ((const short4*)row)[X];
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+
+// pyrUp_<FixPtCast<T,6>> value at level-j pixel (X, Y) from the level-(j+1)
+// source addressed by fetch(Xs, Ys) (border-resolved canvas coordinates):
+// even output column/row taps [1 6 1] at x-1, x, x+1; odd taps [4 4] at x, x+1.
+template <class Fetch>
+__device__ __forceinline__ int4 up_at(const Fetch& fetch, int X, int Y, int cw1, int ch1)
+{
+    int x = X >> 1, y = Y >> 1;
+    int xm = bwrap(x - 1, cw1), xp = bwrap(x + 1, cw1);
+    int ym = breflect(y - 1, ch1), yp = breflect(y + 1, ch1);
+    bool ox = X & 1, oy = Y & 1;
+    auto hrow = [&](int ys) -> int4 {
+        int4 c = fetch(x, ys), r = fetch(xp, ys);
+        if (ox) return make_int4(4 * (c.x + r.x), 4 * (c.y + r.y), 4 * (c.z + r.z), 4 * (c.w + r.w));
+        int4 l = fetch(xm, ys);
+        return make_int4(l.x + 6 * c.x + r.x, l.y + 6 * c.y + r.y, l.z + 6 * c.z + r.z,
+                         l.w + 6 * c.w + r.w);
+    };
+    int4 m = hrow(y), b = hrow(yp), v;
+    if (oy) {
+        v = make_int4(4 * (m.x + b.x), 4 * (m.y + b.y), 4 * (m.z + b.z), 4 * (m.w + b.w));
+    } else {
+        int4 t = hrow(ym);
+        v = make_int4(t.x + 6 * m.x + b.x, t.y + 6 * m.y + b.y, t.z + 6 * m.z + b.z,
+                      t.w + 6 * m.w + b.w);
+    }
+    return make_int4((v.x + 32) >> 6, (v.y + 32) >> 6, (v.z + 32) >> 6, (v.w + 32) >> 6);
+}
+
+// bilinearResampling<short,3> (ZReproject.cpp:101-130) with the gain LUT of
+// transform() (GainAdjust.cpp:295-346) applied to each gathered byte.  The
+// right sample of each row is the next pixel in row-major order (the
+// reference's pointer walk on a continuous image); at the image's last pixel
+// it is the pixel itself.  FP64 products and sums are rounded one at a time in
+// the reference's order; cvRound = round half to even.
+__device__ __forceinline__ uint32_t bilinear_bgr(const uint8_t* src, uint32_t step, int sw, int sh,
+                                                 double x, double y, const uint8_t* lut)
+{
+    int x0 = (int)x, y0 = (int)y;
+    int y1 = y0 + 1 > sh - 1 ? sh - 1 : y0 + 1;
+    double wx0 = __dadd_rn(x, -(double)x0), wx1 = __dadd_rn(1.0, -wx0);
+    double wy0 = __dadd_rn(y, -(double)y0), wy1 = __dadd_rn(1.0, -wy0);
+    double w00 = __dmul_rn(wx1, wy1), w01 = __dmul_rn(wx0, wy1);
+    double w10 = __dmul_rn(wx1, wy0), w11 = __dmul_rn(wx0, wy0);
+    int xn = x0 + 1;
+    int yn0 = y0, yn1 = y1, xn0 = xn, xn1 = xn;
+    if (xn >= sw) {
+        xn0 = xn1 = 0;
+        yn0 = y0 + 1;
+        yn1 = y1 + 1;
+        if (yn0 >= sh) { yn0 = y0; xn0 = x0; }
+        if (yn1 >= sh) { yn1 = y1; xn1 = x0; }
+    }
+    uint32_t p00 = ((const uint32_t*)(src + (size_t)y0 * step))[x0];
+    uint32_t p01 = ((const uint32_t*)(src + (size_t)yn0 * step))[xn0];
+    uint32_t p10 = ((const uint32_t*)(src + (size_t)y1 * step))[x0];
+    uint32_t p11 = ((const uint32_t*)(src + (size_t)yn1 * step))[xn1];
+    uint32_t out = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        int sh8 = 8 * c;
+        int a = (p00 >> sh8) & 255, b = (p01 >> sh8) & 255;
+        int d = (p10 >> sh8) & 255, e = (p11 >> sh8) & 255;
+        if (lut) {
+            a = lut[c * 256 + a];
+            b = lut[c * 256 + b];
+            d = lut[c * 256 + d];
+            e = lut[c * 256 + e];
+        }
+        double r = __dmul_rn((double)a, w00);
+        r = __dadd_rn(r, __dmul_rn((double)b, w01));
+        r = __dadd_rn(r, __dmul_rn((double)d, w10));
+        r = __dadd_rn(r, __dmul_rn((double)e, w11));
+        int v = __double2int_rn(r);
+        out |= (uint32_t)sat_u8(v) << sh8;
+    }
+    return out;
+}
+
+// floor(255 v / (256 a)) for 0 <= v <= 65280, 1 <= a <= 256: t = (255 v) >> 8,
+// then t / a from a reciprocal estimate corrected to the exact quotient.
+__device__ __forceinline__ int div_norm_u8(int v, int a)
+{
+    int t = (255 * v) >> 8;
+    int q = (int)((float)t * __builtin_amdgcn_rcpf((float)a));
+    int r = t - q * a;
+    q += (r >= a) - (r < 0);
+    return q;
+}
+
+
+}  // namespace pano

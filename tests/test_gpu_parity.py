@@ -161,3 +161,29 @@ def test_render_repeatable_and_pitched_output():
         assert r.render(dsrc, view, luts)
     np.testing.assert_array_equal(from_dev(big)[:, :wl.dst_w], got)
     assert not from_dev(big)[:, wl.dst_w:].any()
+
+
+def test_render_inconsistent_maps_use_16bit_planes():
+    # A map that samples where the mask is 0 makes the reference's pyramid leave
+    # [0, 255] (MultiBandBlend.cpp:732-768 divides by the masked alpha only); the
+    # render then keeps 16-bit planes and must still match the oracle.
+    wl = synth.workload("rect6", 0.2)
+    xs, ys, ms = synth.maps_and_masks(wl)
+    ms = [m.copy() for m in ms]
+    rng = np.random.default_rng(5)
+    for m in ms:
+        ys_, xs_ = np.nonzero(m)
+        pick = rng.choice(len(ys_), size=len(ys_) // 50, replace=False)
+        m[ys_[pick], xs_[pick]] = 0
+    srcs = synth.frames(wl)
+    luts = synth.luts(wl)
+    ob = O.Blender()
+    assert ob.prepare(np.stack(ms), wl.levels)
+    ref = ob.render(srcs, xs, ys, luts, nthreads=8)
+    note(f"render inconsistent {wl.dst_w}x{wl.dst_h}")
+    r = P.PanoramaRender()
+    assert r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h),
+                          xs, ys, ms, wl.levels)
+    out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    assert r.render([to_dev(synth.bgrx(s)) for s in srcs], out, luts)
+    assert_parity(from_dev(out)[..., :3], ref, exact=True, label="inconsistent")
