@@ -27,8 +27,12 @@ namespace {
 
 constexpr uint32_t kLo = 0x00ff00ffu;  // (B, R) lanes of a packed pixel; >> 8 gives (G, valid)
 
-// Down tile geometry: input region (2 kFTW + 3) x (2 kFTH + 3) at level j.
+// Down tile geometry: input region (2 kFTW + 3) x (2 kFTH + 3) at level j,
+// staged by kDT threads, kPer pixels each, with all loads of a thread issued
+// before any is used (the loops are latency-bound otherwise).
 constexpr int kIW = 2 * kFTW + 3, kIH = 2 * kFTH + 3;
+constexpr int kDT = 512;
+constexpr int kPer = (kIH * kIW + kDT - 1) / kDT;
 
 // Horizontal 5-tap over the staged input rows, then vertical 5-tap and
 // calcDstImage; writes the (nx, ny) output tile at canvas (X, Y) of plane D.
@@ -36,7 +40,7 @@ __device__ __forceinline__ void down_from_lds(const uint32_t* in, uint2* hs, con
                                               const Plane& D)
 {
     const int ih = 2 * J.ny + 3;
-    for (int e = threadIdx.x; e < kIH * kFTW; e += 256) {
+    for (int e = threadIdx.x; e < kIH * kFTW; e += kDT) {
         int r = e / kFTW, x = e - r * kFTW;
         if (r >= ih || x >= J.nx) continue;
         const uint32_t* p = in + r * kIW + 2 * x;
@@ -47,7 +51,7 @@ __device__ __forceinline__ void down_from_lds(const uint32_t* in, uint2* hs, con
         hs[r * kFTW + x] = make_uint2(lo, hi);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < kFTH * kFTW; e += 256) {
+    for (int e = threadIdx.x; e < kFTH * kFTW; e += kDT) {
         int y = e / kFTW, x = e - y * kFTW;
         if (y >= J.ny || x >= J.nx) continue;
         const uint2* q = hs + (2 * y) * kFTW + x;
@@ -62,14 +66,14 @@ __device__ __forceinline__ void down_from_lds(const uint32_t* in, uint2* hs, con
                 ((uint32_t)div_norm_u8((int)(lo >> 16), al) << 16) | (1u << 24);
         }
         int di = box_index(D, J.X + x, J.Y + y);
-        if (di >= 0) ((uint32_t*)D.g)[di] = o;
+        if (di >= 0) stg((uint32_t*)D.g + di, o);
     }
 }
 
 __device__ __forceinline__ uint32_t load_u32(const Plane& P, int X, int Y)
 {
     int i = box_index(P, X, Y);
-    return i < 0 ? 0u : ((const uint32_t*)P.g)[i];
+    return i < 0 ? 0u : ldg((const uint32_t*)P.g + i);
 }
 
 // pyrUp of a 2x2 quad from packed 8-bit pixels staged in LDS: s points at the
@@ -127,7 +131,7 @@ __device__ __forceinline__ int lane(uint32_t v, int c) { return (int)((v >> (8 *
 // ------------------------------------------------------------ level 0 -> 1 --
 
 template <bool kLut>
-__global__ __launch_bounds__(256) void k_remap_down0(const DownJob* __restrict__ jobs,
+__global__ __launch_bounds__(kDT) void k_remap_down0(const DownJob* __restrict__ jobs,
                                                      const RemapCam* __restrict__ cams,
                                                      const Plane* __restrict__ planes,
                                                      const uint8_t* __restrict__ luts, SrcTable src,
@@ -140,34 +144,65 @@ __global__ __launch_bounds__(256) void k_remap_down0(const DownJob* __restrict__
     const RemapCam cam = cams[J.src];
     const bool use_lut = kLut && cam.lut >= 0;
     if (use_lut)
-        for (int e = threadIdx.x; e < 768; e += 256) lut_s[e] = luts[cam.lut + e];
+        for (int e = threadIdx.x; e < 768; e += kDT) lut_s[e] = luts[cam.lut + e];
     const Plane P0 = planes[cam.plane];
     const Plane D = planes[J.dst];
     const uint8_t* sp = src.p[cam.src];
     const uint32_t sstep = src.step[cam.src];
-    __syncthreads();
     const int bx0 = 2 * J.X - 2, by0 = 2 * J.Y - 2;
     const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
-    for (int e = threadIdx.x; e < kIH * kIW; e += 256) {
-        int r = e / kIW, c = e - r * kIW;
-        if (r >= ih || c >= iw) continue;
-        int vx = bx0 + c, vy = by0 + r;
-        int Xs = bwrap(vx, P0.cw), Ys = breflect(vy, P0.ch);
-        int bi = box_index(P0, Xs, Ys);
-        uint32_t v = 0;
-        if (bi >= 0) {
-            uint32_t fl = cam.mask[bi];
-            float2 m = cam.map[bi];
-            double x = m.x, y = m.y;
-            if (x >= 0 && y >= 0 && x < cam.sw && y < cam.sh)
-                v = bilinear_bgr(sp, sstep, cam.sw, cam.sh, x, y, use_lut ? lut_s : nullptr);
-            if (fl & kFlagValid) v |= 1u << 24;
-            // an owned pixel of this tile's core (not a border-resolved copy)
-            if ((fl & kFlagOwned) && r >= 2 && c >= 2 && r < 2 + 2 * J.ny && c < 2 + 2 * J.nx &&
-                vx == Xs && vy == Ys)
-                g0comp[(size_t)Ys * g0pitch + Xs] = v;
+    // Software pipeline over kRounds rounds of kBatch pixels per thread: the
+    // map points of round r+1 are in flight while round r gathers and combines.
+    constexpr int kBatch = 3, kRounds = (kPer + kBatch - 1) / kBatch;
+    float2 m[2][kBatch];
+    uint32_t fl[2][kBatch];
+    auto fetch_map = [&](int round, float2* mm, uint32_t* ff) {
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) {
+            int e = threadIdx.x + (round * kBatch + b) * kDT;
+            int r = e / kIW, c = e - r * kIW;
+            int bi = -1;
+            if (e < kIH * kIW && r < ih && c < iw)
+                bi = box_index(P0, bwrap(bx0 + c, P0.cw), breflect(by0 + r, P0.ch));
+            mm[b] = bi >= 0 ? ldg_f2(cam.map + bi) : make_float2(-1.f, -1.f);
+            ff[b] = bi >= 0 ? (uint32_t)ldg(cam.mask + bi) : 0u;
         }
-        in[r * kIW + c] = v;
+    };
+    fetch_map(0, m[0], fl[0]);
+    __syncthreads();  // LUT staged
+#pragma unroll
+    for (int round = 0; round < kRounds; round++) {
+        const int cur = round & 1;
+        uint32_t px[kBatch][4];
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) {
+            double x = m[cur][b].x, y = m[cur][b].y;
+            if (x >= 0 && y >= 0 && x < cam.sw && y < cam.sh) {
+                uint32_t off[4];
+                bilinear_offsets(sstep, cam.sw, cam.sh, x, y, off);
+#pragma unroll
+                for (int q = 0; q < 4; q++) px[b][q] = ldg((const uint32_t*)(sp + off[q]));
+            }
+        }
+        if (round + 1 < kRounds) fetch_map(round + 1, m[cur ^ 1], fl[cur ^ 1]);
+#pragma unroll
+        for (int b = 0; b < kBatch; b++) {
+            int e = threadIdx.x + (round * kBatch + b) * kDT;
+            if (e >= kIH * kIW) continue;
+            int r = e / kIW, c = e - r * kIW;
+            uint32_t v = 0;
+            double x = m[cur][b].x, y = m[cur][b].y;
+            if (x >= 0 && y >= 0 && x < cam.sw && y < cam.sh)
+                v = bilinear_combine(x, y, px[b], use_lut ? lut_s : nullptr);
+            if (fl[cur][b] & kFlagValid) v |= 1u << 24;
+            // an owned core pixel (not a border-resolved copy of one) goes to the level-0 blend
+            if ((fl[cur][b] & kFlagOwned) && r >= 2 && c >= 2 && r < 2 + 2 * J.ny &&
+                c < 2 + 2 * J.nx) {
+                int vx = bx0 + c, vy = by0 + r;
+                if (vx < P0.cw && vy < P0.ch) stg(g0comp + (size_t)vy * g0pitch + vx, v);
+            }
+            in[e] = v;
+        }
     }
     __syncthreads();
     down_from_lds(in, hs, J, D);
@@ -175,7 +210,7 @@ __global__ __launch_bounds__(256) void k_remap_down0(const DownJob* __restrict__
 
 // ------------------------------------------------------- level j -> j+1 --
 
-__global__ __launch_bounds__(256) void k_down_u8(const DownJob* __restrict__ jobs,
+__global__ __launch_bounds__(kDT) void k_down_u8(const DownJob* __restrict__ jobs,
                                                  const Plane* __restrict__ planes)
 {
     __shared__ uint32_t in[kIH * kIW];
@@ -185,10 +220,19 @@ __global__ __launch_bounds__(256) void k_down_u8(const DownJob* __restrict__ job
     const Plane D = planes[J.dst];
     const int bx0 = 2 * J.X - 2, by0 = 2 * J.Y - 2;
     const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
-    for (int e = threadIdx.x; e < kIH * kIW; e += 256) {
+    uint32_t v[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        int e = threadIdx.x + k * kDT;
         int r = e / kIW, c = e - r * kIW;
-        if (r >= ih || c >= iw) continue;
-        in[r * kIW + c] = load_u32(S, bwrap(bx0 + c, S.cw), breflect(by0 + r, S.ch));
+        v[k] = 0;
+        if (e < kIH * kIW && r < ih && c < iw)
+            v[k] = load_u32(S, bwrap(bx0 + c, S.cw), breflect(by0 + r, S.ch));
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; k++) {
+        int e = threadIdx.x + k * kDT;
+        if (e < kIH * kIW) in[e] = v[k];
     }
     __syncthreads();
     down_from_lds(in, hs, J, D);
@@ -204,7 +248,7 @@ __global__ __launch_bounds__(256) void k_blend_u8(const TileJob* __restrict__ jo
                                                   const CanvasLevel* __restrict__ canvas,
                                                   const RenderDesc* __restrict__ rds, int level)
 {
-    __shared__ uint32_t gs[kSH * kSW];
+    __shared__ uint32_t gs[kMaxStage][kSH * kSW];
     __shared__ int4 cs[kSH * kSW];
     const TileJob J = jobs[blockIdx.x];
     const RenderDesc R = rds[J.r];
@@ -212,64 +256,106 @@ __global__ __launch_bounds__(256) void k_blend_u8(const TileJob* __restrict__ jo
     const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
     const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
     const int sx0 = (J.X >> 1) - 1, sy0 = (J.Y >> 1) - 1;
+    const CanvasLevel C = canvas[R.canvas0 + level];
     bool in[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) in[d] = X + (d & 1) < J.X + J.nx && Y + (d >> 1) < J.Y + J.ny;
-    int acc[4][3] = {};
-    for (int k = 0; k < J.cnt; k++) {
-        const int pi = camlist[J.off + k];
-        const Plane P = planes[pi];
-        if (!top) {
-            const Plane P1 = planes[pi + 1];
-            for (int e = threadIdx.x; e < kSH * kSW; e += 256) {
-                int r = e / kSW, c = e - r * kSW;
-                gs[e] = load_u32(P1, bwrap(sx0 + c, P1.cw), breflect(sy0 + r, P1.ch));
-            }
-            __syncthreads();
-        }
-        uint32_t u[4] = {0, 0, 0, 0};
-        if (!top) up_quad_u8(gs + (qy + 1) * kSW + qx + 1, kSW, u);
+    int ws[4] = {1, 1, 1, 1};
+    if (!R.full)
 #pragma unroll
-        for (int d = 0; d < 4; d++) {
-            if (!in[d]) continue;
-            int i = box_index(P, X + (d & 1), Y + (d >> 1));
-            if (i < 0) continue;
-            int w = P.w[i];
-            if (!w) continue;
-            uint32_t g = ((const uint32_t*)P.g)[i];
-#pragma unroll
-            for (int c = 0; c < 3; c++) acc[d][c] += (lane(g, c) - lane(u[d], c)) * w;
-        }
-        if (!top) __syncthreads();
-    }
-    const CanvasLevel C = canvas[R.canvas0 + level];
-    int3 up[4] = {};
+        for (int d = 0; d < 4; d++)
+            if (in[d]) ws[d] = ldg(C.wsum + (Y + (d >> 1)) * C.pitch + X + (d & 1));
+    // stage up to kMaxStage cameras' level j+1 regions and C_{j+1} with every load in flight
+    constexpr int kStageIt = (kSH * kSW + 255) / 256;
     if (!top) {
         const CanvasLevel C1 = canvas[R.canvas0 + level + 1];
-        for (int e = threadIdx.x; e < kSH * kSW; e += 256) {
+        int4 cv[kStageIt];
+#pragma unroll
+        for (int k = 0; k < kStageIt; k++) {
+            int e = threadIdx.x + k * 256;
             int r = e / kSW, c = e - r * kSW;
-            cs[e] = C1.c[breflect(sy0 + r, C1.ch) * C1.pitch + bwrap(sx0 + c, C1.cw)];
+            cv[k] = e < kSH * kSW
+                        ? ldg4(C1.c + breflect(sy0 + r, C1.ch) * C1.pitch + bwrap(sx0 + c, C1.cw))
+                        : make_int4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < kStageIt; k++) {
+            int e = threadIdx.x + k * 256;
+            if (e < kSH * kSW) cs[e] = cv[k];
+        }
+    }
+    int3 acc[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) acc[d] = make_int3(0, 0, 0);
+    for (int g = 0; g < J.cnt; g += kMaxStage) {
+        const int ng = J.cnt - g < kMaxStage ? J.cnt - g : kMaxStage;
+        if (!top) {
+            uint32_t gv[kMaxStage][kStageIt];
+#pragma unroll
+            for (int s = 0; s < kMaxStage; s++) {
+                const Plane P1 = planes[camlist[J.off + g + (s < ng ? s : 0)] + 1];
+#pragma unroll
+                for (int k = 0; k < kStageIt; k++) {
+                    int e = threadIdx.x + k * 256;
+                    int r = e / kSW, c = e - r * kSW;
+                    gv[s][k] = (s < ng && e < kSH * kSW)
+                                   ? load_u32(P1, bwrap(sx0 + c, P1.cw), breflect(sy0 + r, P1.ch))
+                                   : 0u;
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < kMaxStage; s++)
+#pragma unroll
+                for (int k = 0; k < kStageIt; k++) {
+                    int e = threadIdx.x + k * 256;
+                    if (s < ng && e < kSH * kSW) gs[s][e] = gv[s][k];
+                }
         }
         __syncthreads();
+        for (int s = 0; s < ng; s++) {
+            const Plane P = planes[camlist[J.off + g + s]];
+            int idx[4], w[4];
+            uint32_t gp[4];
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                idx[d] = in[d] ? box_index(P, X + (d & 1), Y + (d >> 1)) : -1;
+                w[d] = idx[d] >= 0 ? (int)ldg(P.w + idx[d]) : 0;
+                gp[d] = idx[d] >= 0 ? ldg((const uint32_t*)P.g + idx[d]) : 0u;
+            }
+            uint32_t u[4] = {0, 0, 0, 0};
+            if (!top) up_quad_u8(&gs[s][(qy + 1) * kSW + qx + 1], kSW, u);
+#pragma unroll
+            for (int d = 0; d < 4; d++) {
+                acc[d].x += (lane(gp[d], 0) - lane(u[d], 0)) * w[d];
+                acc[d].y += (lane(gp[d], 1) - lane(u[d], 1)) * w[d];
+                acc[d].z += (lane(gp[d], 2) - lane(u[d], 2)) * w[d];
+            }
+        }
+        __syncthreads();
+    }
+    int3 up[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) up[d] = make_int3(0, 0, 0);
+    if (!top) {
+        if (J.cnt == 0) __syncthreads();  // cs staged, no camera loop ran
         up_quad_i32(cs + (qy + 1) * kSW + qx + 1, kSW, up);
     }
 #pragma unroll
     for (int d = 0; d < 4; d++) {
         if (!in[d]) continue;
         int Xp = X + (d & 1), Yp = Y + (d >> 1);
-        int a0 = acc[d][0], a1 = acc[d][1], a2 = acc[d][2];
+        int a0 = acc[d].x, a1 = acc[d].y, a2 = acc[d].z;
         if (R.full) {
             a0 >>= 8;
             a1 >>= 8;
             a2 >>= 8;
         } else {
-            int ws = C.wsum[Yp * C.pitch + Xp];
-            if (!ws) ws = 1;
-            a0 /= ws;
-            a1 /= ws;
-            a2 /= ws;
+            int wv = ws[d] ? ws[d] : 1;
+            a0 /= wv;
+            a1 /= wv;
+            a2 /= wv;
         }
-        C.c[Yp * C.pitch + Xp] = make_int4(up[d].x + a0, up[d].y + a1, up[d].z + a2, 0);
+        stg4(C.c + Yp * C.pitch + Xp, make_int4(up[d].x + a0, up[d].y + a1, up[d].z + a2, 0));
     }
 }
 
@@ -291,29 +377,66 @@ __global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jo
     const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
     const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
     const int sx0 = (J.X >> 1) - 1, sy0 = (J.Y >> 1) - 1;
+    constexpr int kStageIt = (kSH * kSW + 255) / 256;
     bool in[4];
     int lab[4];
     uint32_t g0[4];
-    int lap[4][3];
 #pragma unroll
     for (int d = 0; d < 4; d++) {
         int Xp = X + (d & 1), Yp = Y + (d >> 1);
         in[d] = Xp < J.X + J.nx && Yp < J.Y + J.ny;
-        lab[d] = in[d] ? R.label[Yp * R.label_pitch + Xp] : kNoOwner;
-        g0[d] = lab[d] != kNoOwner ? g0comp[(size_t)Yp * g0pitch + Xp] : 0u;
+        lab[d] = in[d] ? (int)ldg(R.label + Yp * R.label_pitch + Xp) : (int)kNoOwner;
+        g0[d] = in[d] ? g0comp[(size_t)Yp * g0pitch + Xp] : 0u;
+    }
+    if (L > 0) {
+        const CanvasLevel C1 = canvas[R.canvas0 + 1];
+        int4 cv[kStageIt];
+#pragma unroll
+        for (int k = 0; k < kStageIt; k++) {
+            int e = threadIdx.x + k * 256;
+            int r = e / kSW, c = e - r * kSW;
+            cv[k] = e < kSH * kSW
+                        ? ldg4(C1.c + breflect(sy0 + r, C1.ch) * C1.pitch + bwrap(sx0 + c, C1.cw))
+                        : make_int4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int k = 0; k < kStageIt; k++) {
+            int e = threadIdx.x + k * 256;
+            if (e < kSH * kSW) cs[e] = cv[k];
+        }
+    }
+    int lap[4][3];
+#pragma unroll
+    for (int d = 0; d < 4; d++)
 #pragma unroll
         for (int c = 0; c < 3; c++) lap[d][c] = lane(g0[d], c);
-    }
-    int3 up[4] = {};
+    int3 up[4];
+#pragma unroll
+    for (int d = 0; d < 4; d++) up[d] = make_int3(0, 0, 0);
     if (L > 0) {
         for (int g = 0; g < J.cnt; g += kMaxStage) {
             const int ng = J.cnt - g < kMaxStage ? J.cnt - g : kMaxStage;
-            for (int e = threadIdx.x; e < ng * kSH * kSW; e += 256) {
-                int s = e / (kSH * kSW), k = e - s * (kSH * kSW);
-                int r = k / kSW, c = k - r * kSW;
-                const Plane P1 = planes[R.plane0 + owners[J.off + g + s] * (L + 1) + 1];
-                gs[s][k] = load_u32(P1, bwrap(sx0 + c, P1.cw), breflect(sy0 + r, P1.ch));
+            uint32_t gv[kMaxStage][kStageIt];
+#pragma unroll
+            for (int s = 0; s < kMaxStage; s++) {
+                const int o = owners[J.off + g + (s < ng ? s : 0)];
+                const Plane P1 = planes[R.plane0 + o * (L + 1) + 1];
+#pragma unroll
+                for (int k = 0; k < kStageIt; k++) {
+                    int e = threadIdx.x + k * 256;
+                    int r = e / kSW, c = e - r * kSW;
+                    gv[s][k] = (s < ng && e < kSH * kSW)
+                                   ? load_u32(P1, bwrap(sx0 + c, P1.cw), breflect(sy0 + r, P1.ch))
+                                   : 0u;
+                }
             }
+#pragma unroll
+            for (int s = 0; s < kMaxStage; s++)
+#pragma unroll
+                for (int k = 0; k < kStageIt; k++) {
+                    int e = threadIdx.x + k * 256;
+                    if (s < ng && e < kSH * kSW) gs[s][e] = gv[s][k];
+                }
             __syncthreads();
             for (int s = 0; s < ng; s++) {
                 const int o = owners[J.off + g + s];
@@ -328,12 +451,7 @@ __global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jo
             }
             __syncthreads();
         }
-        const CanvasLevel C1 = canvas[R.canvas0 + 1];
-        for (int e = threadIdx.x; e < kSH * kSW; e += 256) {
-            int r = e / kSW, c = e - r * kSW;
-            cs[e] = C1.c[breflect(sy0 + r, C1.ch) * C1.pitch + bwrap(sx0 + c, C1.cw)];
-        }
-        __syncthreads();
+        if (J.cnt == 0) __syncthreads();  // cs staged, no owner loop ran
         up_quad_i32(cs + (qy + 1) * kSW + qx + 1, kSW, up);
     }
     uint8_t* out = outs.p[R.out];
@@ -346,7 +464,7 @@ __global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jo
         if (R.full || lab[d] != kNoOwner)
             o = (uint32_t)sat_u8(up[d].x + lap[d][0]) | ((uint32_t)sat_u8(up[d].y + lap[d][1]) << 8) |
                 ((uint32_t)sat_u8(up[d].z + lap[d][2]) << 16);
-        ((uint32_t*)(out + (size_t)Yp * ostep))[Xp] = o;
+        stg((uint32_t*)(out + (size_t)Yp * ostep) + Xp, o);
     }
 }
 
@@ -358,10 +476,10 @@ hipError_t launch_remap_down0(const DownJob* jobs, int njobs, const RemapCam* ca
 {
     if (njobs <= 0) return hipSuccess;
     if (use_lut)
-        hipLaunchKernelGGL(k_remap_down0<true>, dim3(njobs), dim3(256), 0, s, jobs, cams, planes,
+        hipLaunchKernelGGL(k_remap_down0<true>, dim3(njobs), dim3(kDT), 0, s, jobs, cams, planes,
                            luts, src, g0comp, g0pitch);
     else
-        hipLaunchKernelGGL(k_remap_down0<false>, dim3(njobs), dim3(256), 0, s, jobs, cams, planes,
+        hipLaunchKernelGGL(k_remap_down0<false>, dim3(njobs), dim3(kDT), 0, s, jobs, cams, planes,
                            luts, src, g0comp, g0pitch);
     return hipGetLastError();
 }
@@ -369,7 +487,7 @@ hipError_t launch_remap_down0(const DownJob* jobs, int njobs, const RemapCam* ca
 hipError_t launch_down_u8(const DownJob* jobs, int njobs, const Plane* planes, hipStream_t s)
 {
     if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_down_u8, dim3(njobs), dim3(256), 0, s, jobs, planes);
+    hipLaunchKernelGGL(k_down_u8, dim3(njobs), dim3(kDT), 0, s, jobs, planes);
     return hipGetLastError();
 }
 

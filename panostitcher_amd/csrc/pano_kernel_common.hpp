@@ -10,6 +10,44 @@ namespace pano {
 
 // ---------------------------------------------------------------- helpers --
 
+// Loads and stores through pointers held in descriptors: the compiler cannot
+// see that those point to global memory and would emit flat instructions.
+#define PANO_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T ldg(const T* p)
+{
+    return *(const PANO_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ void stg(T* p, T v)
+{
+    *(PANO_GLOBAL T*)p = v;
+}
+// HIP's vector structs do not take the address-space cast as one access; go
+// through the compiler's native vector types.
+typedef int pano_i32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int4 ldg4(const int4* p)
+{
+    pano_i32x4 v = *(const PANO_GLOBAL pano_i32x4*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void stg4(int4* p, int4 v)
+{
+    pano_i32x4 w = {v.x, v.y, v.z, v.w};
+    *(PANO_GLOBAL pano_i32x4*)p = w;
+}
+typedef short pano_i16x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ short4 ldg_s4(const short4* p)
+{
+    pano_i16x4 v = *(const PANO_GLOBAL pano_i16x4*)p;
+    return make_short4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 ldg_f2(const float2* p)
+{
+    unsigned long long v = *(const PANO_GLOBAL unsigned long long*)p;
+    return make_float2(__uint_as_float((uint32_t)v), __uint_as_float((uint32_t)(v >> 32)));
+}
+
 // cv::borderInterpolate for WRAP (columns) and REFLECT_101 (rows).
 __device__ __forceinline__ int bwrap(int p, int n)
 {
@@ -51,8 +89,8 @@ __device__ __forceinline__ int sat_u8(int v) { return v < 0 ? 0 : (v > 255 ? 255
 template <int K>
 __device__ __forceinline__ int4 load_at(const void* g, int i)
 {
-    if (K == kPixU8) return unpack_u8(((const uint32_t*)g)[i]);
-    short4 s = ((const short4*)g)[i];
+    if (K == kPixU8) return unpack_u8(ldg((const uint32_t*)g + i));
+    short4 s = ldg_s4((const short4*)g + i);
     return make_int4(s.x, s.y, s.z, s.w);
 }
 
@@ -70,8 +108,8 @@ template <int K>
 __device__ __forceinline__ int4 load_user(const SrcTable& src, int s, int X, int Y)
 {
     const uint8_t* row = src.p[s] + (size_t)Y * src.step[s];
-    if (K == kPixU8) return unpack_u8(((const uint32_t*)row)[X]);
-    short4 v = ((const short4*)row)[X];
+    if (K == kPixU8) return unpack_u8(ldg((const uint32_t*)row + X));
+    short4 v = ldg_s4((const short4*)row + X);
     return make_int4(v.x, v.y, v.z, v.w);
 }
 
@@ -103,40 +141,46 @@ __device__ __forceinline__ int4 up_at(const Fetch& fetch, int X, int Y, int cw1,
     return make_int4((v.x + 32) >> 6, (v.y + 32) >> 6, (v.z + 32) >> 6, (v.w + 32) >> 6);
 }
 
-// bilinearResampling<short,3> (ZReproject.cpp:101-130) with the gain LUT of
-// transform() (GainAdjust.cpp:295-346) applied to each gathered byte.  The
-// right sample of each row is the next pixel in row-major order (the
-// reference's pointer walk on a continuous image); at the image's last pixel
-// it is the pixel itself.  FP64 products and sums are rounded one at a time in
-// the reference's order; cvRound = round half to even.
-__device__ __forceinline__ uint32_t bilinear_bgr(const uint8_t* src, uint32_t step, int sw, int sh,
-                                                 double x, double y, const uint8_t* lut)
+// Byte offsets (from the image base) of the four samples of
+// bilinearResampling at a valid point (x, y): (x0, y0), its right neighbour in
+// row-major order, (x0, y1) and the right neighbour of that.
+__device__ __forceinline__ void bilinear_offsets(uint32_t step, int sw, int sh, double x, double y,
+                                                 uint32_t off[4])
 {
     int x0 = (int)x, y0 = (int)y;
     int y1 = y0 + 1 > sh - 1 ? sh - 1 : y0 + 1;
-    double wx0 = __dadd_rn(x, -(double)x0), wx1 = __dadd_rn(1.0, -wx0);
-    double wy0 = __dadd_rn(y, -(double)y0), wy1 = __dadd_rn(1.0, -wy0);
-    double w00 = __dmul_rn(wx1, wy1), w01 = __dmul_rn(wx0, wy1);
-    double w10 = __dmul_rn(wx1, wy0), w11 = __dmul_rn(wx0, wy0);
-    int xn = x0 + 1;
-    int yn0 = y0, yn1 = y1, xn0 = xn, xn1 = xn;
-    if (xn >= sw) {
+    int xn0 = x0 + 1, xn1 = x0 + 1, yn0 = y0, yn1 = y1;
+    if (x0 + 1 >= sw) {
         xn0 = xn1 = 0;
         yn0 = y0 + 1;
         yn1 = y1 + 1;
         if (yn0 >= sh) { yn0 = y0; xn0 = x0; }
         if (yn1 >= sh) { yn1 = y1; xn1 = x0; }
     }
-    uint32_t p00 = ((const uint32_t*)(src + (size_t)y0 * step))[x0];
-    uint32_t p01 = ((const uint32_t*)(src + (size_t)yn0 * step))[xn0];
-    uint32_t p10 = ((const uint32_t*)(src + (size_t)y1 * step))[x0];
-    uint32_t p11 = ((const uint32_t*)(src + (size_t)yn1 * step))[xn1];
+    off[0] = (uint32_t)y0 * step + 4u * (uint32_t)x0;
+    off[1] = (uint32_t)yn0 * step + 4u * (uint32_t)xn0;
+    off[2] = (uint32_t)y1 * step + 4u * (uint32_t)x0;
+    off[3] = (uint32_t)yn1 * step + 4u * (uint32_t)xn1;
+}
+
+// The four gathered BGRx samples combined with the weights of (x, y): the gain
+// LUT of transform() (GainAdjust.cpp:295-346) applied to each byte, products
+// and sums rounded one at a time in the reference's order (p00 w00 + p01 w01
+// + p10 w10 + p11 w11), cvRound = round half to even.
+__device__ __forceinline__ uint32_t bilinear_combine(double x, double y, const uint32_t p[4],
+                                                     const uint8_t* lut)
+{
+    int x0 = (int)x, y0 = (int)y;
+    double wx0 = __dadd_rn(x, -(double)x0), wx1 = __dadd_rn(1.0, -wx0);
+    double wy0 = __dadd_rn(y, -(double)y0), wy1 = __dadd_rn(1.0, -wy0);
+    double w00 = __dmul_rn(wx1, wy1), w01 = __dmul_rn(wx0, wy1);
+    double w10 = __dmul_rn(wx1, wy0), w11 = __dmul_rn(wx0, wy0);
     uint32_t out = 0;
 #pragma unroll
     for (int c = 0; c < 3; c++) {
         int sh8 = 8 * c;
-        int a = (p00 >> sh8) & 255, b = (p01 >> sh8) & 255;
-        int d = (p10 >> sh8) & 255, e = (p11 >> sh8) & 255;
+        int a = (p[0] >> sh8) & 255, b = (p[1] >> sh8) & 255;
+        int d = (p[2] >> sh8) & 255, e = (p[3] >> sh8) & 255;
         if (lut) {
             a = lut[c * 256 + a];
             b = lut[c * 256 + b];
@@ -147,10 +191,23 @@ __device__ __forceinline__ uint32_t bilinear_bgr(const uint8_t* src, uint32_t st
         r = __dadd_rn(r, __dmul_rn((double)b, w01));
         r = __dadd_rn(r, __dmul_rn((double)d, w10));
         r = __dadd_rn(r, __dmul_rn((double)e, w11));
-        int v = __double2int_rn(r);
-        out |= (uint32_t)sat_u8(v) << sh8;
+        out |= (uint32_t)sat_u8(__double2int_rn(r)) << sh8;
     }
     return out;
+}
+
+// bilinearResampling<short,3> (ZReproject.cpp:101-130) with the LUT: the
+// right sample of each row is the next pixel in row-major order (the
+// reference's pointer walk on a continuous image, which at x0 = w - 1 is the
+// first pixel of the next row); at the image's last pixel it is the pixel itself.
+__device__ __forceinline__ uint32_t bilinear_bgr(const uint8_t* src, uint32_t step, int sw, int sh,
+                                                 double x, double y, const uint8_t* lut)
+{
+    uint32_t off[4], p[4];
+    bilinear_offsets(step, sw, sh, x, y, off);
+#pragma unroll
+    for (int k = 0; k < 4; k++) p[k] = ldg((const uint32_t*)(src + off[k]));
+    return bilinear_combine(x, y, p, lut);
 }
 
 // floor(255 v / (256 a)) for 0 <= v <= 65280, 1 <= a <= 256: t = (255 v) >> 8,
