@@ -118,6 +118,26 @@ struct PinnedRing {
     }
 };
 
+int host_wrap(int p, int n)
+{
+    p %= n;
+    return p < 0 ? p + n : p;
+}
+
+int host_reflect(int p, int n)
+{
+    if ((unsigned)p < (unsigned)n) return p;
+    if (n == 1) return 0;
+    do {
+        p = p < 0 ? -p : 2 * n - 2 - p;
+    } while ((unsigned)p >= (unsigned)n);
+    return p;
+}
+
+// Margined map storage of a level-0 box (fused path).
+int map_pitch(const Geom& g) { return g.bw > 0 ? (g.bw + 2 * kMapMargin + 63) / 64 * 64 : 0; }
+size_t map_cells(const Geom& g) { return g.bh > 0 ? (size_t)map_pitch(g) * (g.bh + 2 * kMapMargin) : 0; }
+
 template <class T>
 T* upload_vec(Arena& a, const std::vector<T>& v, int& err)
 {
@@ -197,8 +217,9 @@ struct DeviceRig {
             pl.add(sizeof(RemapCam) * n);
             for (int i = 0; i < n; i++) {
                 const Geom& g = prep.geom[(size_t)i * (L + 1)];
-                pl.add(g.pixels() * 8);
-                pl.add(g.pixels());
+                size_t cells = fused ? map_cells(g) : g.pixels();
+                pl.add(cells * 8);
+                pl.add(cells);
             }
             pl.add((size_t)n * 768);
         }
@@ -273,6 +294,19 @@ struct DeviceRig {
         rd.canvas0 = 0;
         rd.out = 0;
         d_rd = upload_vec(arena, std::vector<RenderDesc>{rd}, err);
+        if (fused && L > 0) {
+            for (DownJob& d : prep.down[0]) {
+                const Geom& g = prep.geom[(size_t)d.src * (L + 1)];
+                // staging region: level-0 columns 2X - 2 .., rows 2Y - 2 ..
+                int mx = host_wrap(2 * d.X - 2 - (g.x0 - kMapMargin), g.cw);
+                int my = 2 * d.Y - 2 - (g.y0 - kMapMargin);
+                if (my < 0 || my + 2 * d.ny + 3 > g.bh + 2 * kMapMargin ||
+                    mx + 2 * d.nx + 3 > g.bw + 2 * kMapMargin)
+                    return PANO_ERR_ARG;  // a staging region outside the margined box
+                d.mx = mx;
+                d.my = my;
+            }
+        }
         d_down.assign(L, nullptr);
         n_down.assign(L, 0);
         for (int j = 0; j < L; j++) {
@@ -305,25 +339,35 @@ struct DeviceRig {
                 c.src = i;
                 c.lut = i * 768;
                 c.plane = i * (L + 1);
-                float2* dm = (float2*)arena.take(g.pixels() * 8);
-                uint8_t* dk = (uint8_t*)arena.take(g.pixels());
-                std::vector<float2> hm(g.pixels(), make_float2(-1.f, -1.f));
-                std::vector<uint8_t> hk(g.pixels(), 0);
-                for (int ly = 0; ly < g.bh; ly++)
-                    for (int lx = 0; lx < g.bw; lx++) {
-                        int X = (g.x0 + lx) % g.cw, Y = g.y0 + ly;
+                const int m = fused ? kMapMargin : 0;
+                const int mp = fused ? map_pitch(g) : g.pitch;
+                const int mrows = g.bh > 0 ? g.bh + 2 * m : 0;
+                const size_t cells = (size_t)mp * mrows;
+                float2* dm = (float2*)arena.take(cells * 8);
+                uint8_t* dk = (uint8_t*)arena.take(cells);
+                std::vector<float2> hm(cells, make_float2(-1.f, -1.f));
+                std::vector<uint8_t> hk(cells, 0);
+                // entry (sy, sx) holds canvas (x0 - m + sx, y0 - m + sy) after border
+                // resolution, or nothing when that lies outside the box
+                for (int sy = 0; sy < mrows; sy++)
+                    for (int sx = 0; sx < g.bw + 2 * m && g.bw > 0; sx++) {
+                        int X = host_wrap(g.x0 - m + sx, g.cw), Y = host_reflect(g.y0 - m + sy, g.ch);
+                        int ly = Y - g.y0, lx = X - g.x0;
+                        if (lx < 0) lx += g.cw;
+                        if (ly < 0 || ly >= g.bh || lx >= g.bw) continue;
                         size_t o = (size_t)Y * prep.cols + X;
-                        hm[(size_t)ly * g.pitch + lx] = make_float2(xmaps[i][o], ymaps[i][o]);
+                        hm[(size_t)sy * mp + sx] = make_float2(xmaps[i][o], ymaps[i][o]);
                         uint8_t f = masks[i][(size_t)Y * mstep[i] + X] ? kFlagValid : 0;
                         if (prep.label[o] == i) f |= kFlagOwned;
-                        hk[(size_t)ly * g.pitch + lx] = f;
+                        hk[(size_t)sy * mp + sx] = f;
                     }
-                if (g.pixels()) {
-                    HIP_TRY(hipMemcpy(dm, hm.data(), g.pixels() * 8, hipMemcpyHostToDevice));
-                    HIP_TRY(hipMemcpy(dk, hk.data(), g.pixels(), hipMemcpyHostToDevice));
+                if (cells) {
+                    HIP_TRY(hipMemcpy(dm, hm.data(), cells * 8, hipMemcpyHostToDevice));
+                    HIP_TRY(hipMemcpy(dk, hk.data(), cells, hipMemcpyHostToDevice));
                 }
                 c.map = dm;
                 c.mask = dk;
+                c.mpitch = mp;
             }
             d_cams = upload_vec(arena, cams, err);
             d_luts = (uint8_t*)arena.take((size_t)n * 768);

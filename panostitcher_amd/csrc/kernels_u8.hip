@@ -161,11 +161,10 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const DownJob* __restrict__
         for (int b = 0; b < kBatch; b++) {
             int e = threadIdx.x + (round * kBatch + b) * kDT;
             int r = e / kIW, c = e - r * kIW;
-            int bi = -1;
-            if (e < kIH * kIW && r < ih && c < iw)
-                bi = box_index(P0, bwrap(bx0 + c, P0.cw), breflect(by0 + r, P0.ch));
-            mm[b] = bi >= 0 ? ldg_f2(cam.map + bi) : make_float2(-1.f, -1.f);
-            ff[b] = bi >= 0 ? (uint32_t)ldg(cam.mask + bi) : 0u;
+            bool ok = e < kIH * kIW && r < ih && c < iw;
+            int bi = (J.my + r) * cam.mpitch + J.mx + c;
+            mm[b] = ok ? ldg_f2(cam.map + bi) : make_float2(-1.f, -1.f);
+            ff[b] = ok ? (uint32_t)ldg(cam.mask + bi) : 0u;
         }
     };
     fetch_map(0, m[0], fl[0]);

@@ -70,19 +70,26 @@ struct RenderDesc {   // per render (rig) of a launch
     int out;               // OutTable index of its output canvas
 };
 
-struct RemapCam {     // per camera of the fused path (resident)
-    const float2* map;     // box-local (x, y), pitch = level-0 plane pitch
-    const uint8_t* mask;   // box-local flags (kFlagValid | kFlagOwned), pitch = level-0 plane pitch
+// Per camera of the render path (resident).  map/mask cover the camera's
+// level-0 box.  For the 8-bit fused path they carry a kMapMargin frame of
+// border-resolved copies (WRAP columns, REFLECT_101 rows, invalid outside the
+// box) so that a tile's staging region is one contiguous rectangle; for the
+// 16-bit path they are box-local with the level-0 plane pitch.
+constexpr int kMapMargin = 5;
+struct RemapCam {
+    const float2* map;     // (x, y) map points
+    const uint8_t* mask;   // flags (kFlagValid | kFlagOwned)
     int sw, sh;            // source image size
     int src;               // SrcTable index of its frame
     int lut;               // byte offset of its 3 x 256 LUT in the LUT buffer, -1 = none
     int plane;             // its level-0 plane
-    int pad;
+    int mpitch;            // elements per row of map and mask
 };
 
 struct RemapJob { int cam, lx, ly, nx, ny, pad0, pad1, pad2; };   // box-local tile, level 0
-struct DownJob { int src, dst, X, Y, nx, ny, pad0, pad1; };       // canvas tile at level j+1
-                                                                  // (fused level 0: src = camera)
+// Canvas tile at level j+1.  Fused level 0: src = camera, and (mx, my) is the
+// position of the tile's staging region in the camera's margined map storage.
+struct DownJob { int src, dst, X, Y, nx, ny, mx, my; };
 struct TileJob { int X, Y, nx, ny, r, off, cnt, flags; };         // canvas tile at level j
 
 enum TileFlags : int { kTileTop = 1 };
