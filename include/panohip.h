@@ -173,6 +173,32 @@ int64_t pano_render_mem_bytes(const pano_render* r);
  * 3 level-0 blend, 4 total. */
 int64_t pano_render_model_bytes(const pano_render* r, int stage);
 
+/* 1 when the render runs the fused 8-bit path (its maps sample only inside
+ * their masks and it has at least one pyramid level), 0 for the 16-bit path. */
+int pano_render_fused(const pano_render* r);
+
+/* ---- batched render: several rigs per launch ------------------------------
+ * One launch per pyramid stage covers every camera of every render in the
+ * batch, each render's tiles taken from its own resident work lists (no
+ * counterpart in the reference, which stitches one rig per call,
+ * RicohUtil.cpp:1207-1339; this is for servers that stitch many streams).
+ * Up to 32 renders and 128 cameras in total, all on the fused 8-bit path.
+ * The renders must outlive the batch.  pano_batch_set waits for the batch's
+ * earlier work before replacing its tables. */
+typedef struct pano_batch pano_batch;
+
+int pano_batch_create(pano_batch** out);
+void pano_batch_destroy(pano_batch* b);
+int pano_batch_set(pano_batch* b, int num, pano_render* const* renders);
+/* srcs/src_steps: the cameras of render 0, then render 1, ... (device BGRx);
+ * luts: per render a host n x 3 x 256 block or NULL (no correction), or NULL
+ * for none at all; dsts/dst_steps: per render a device BGRx canvas. */
+int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                      const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                      hipStream_t stream);
+int pano_batch_set_timing(pano_batch* b, int enable);
+int pano_batch_stage_ms(pano_batch* b, double* ms4, int* frames);
+
 /* Per-stage device timing of render() (no counterpart in the reference, whose
  * timers are host-side, Tool/Timer.h:8-143): when enabled, HIP events are
  * recorded on the render's stream between its stages.  pano_render_stage_ms

@@ -90,6 +90,13 @@ def lib():
             "pano_render_model_bytes": ([vp, i], i64),
             "pano_render_set_timing": ([vp, i], i),
             "pano_render_stage_ms": ([vp, vp, C.POINTER(C.c_int)], i),
+            "pano_render_fused": ([vp], i),
+            "pano_batch_create": ([pp], i),
+            "pano_batch_destroy": ([vp], None),
+            "pano_batch_set": ([vp, i, pp], i),
+            "pano_batch_render": ([vp, pp, psz, pp, pp, psz, vp], i),
+            "pano_batch_set_timing": ([vp, i], i),
+            "pano_batch_stage_ms": ([vp, vp, C.POINTER(C.c_int)], i),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
@@ -110,6 +117,8 @@ def exported_symbols():
         "pano_render_prepare", "pano_render_prepare_maps", "pano_render_render",
         "pano_render_num_images", "pano_render_num_levels", "pano_render_mem_bytes",
         "pano_render_model_bytes", "pano_render_set_timing", "pano_render_stage_ms",
+        "pano_render_fused", "pano_batch_create", "pano_batch_destroy", "pano_batch_set",
+        "pano_batch_render", "pano_batch_set_timing", "pano_batch_stage_ms",
     ]
 
 
@@ -370,6 +379,10 @@ class PanoramaRender:
     def num_levels(self):
         return lib().pano_render_num_levels(self._h)
 
+    @property
+    def fused(self) -> bool:
+        return bool(lib().pano_render_fused(self._h))
+
     def mem_bytes(self) -> int:
         return lib().pano_render_mem_bytes(self._h)
 
@@ -403,4 +416,61 @@ class PanoramaRender:
         if st == -2:
             return False
         _check(st, "render")
+        return True
+
+
+class Batch:
+    """Several prepared renders stitched with one launch per pyramid stage
+    (pano_batch_*).  The renders must stay alive while the batch uses them."""
+
+    def __init__(self, renders=None):
+        h = C.c_void_p()
+        _check(lib().pano_batch_create(C.byref(h)), "create batch")
+        self._h = h
+        self.renders = []
+        if renders:
+            self.set(renders)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pano_batch_destroy(h)
+            self._h = None
+
+    def set(self, renders):
+        _check(lib().pano_batch_set(self._h, len(renders), _ptrs([r._h.value for r in renders])),
+               "batch set")
+        self.renders = list(renders)
+
+    STAGES = PanoramaRender.STAGES
+
+    def set_timing(self, on: bool):
+        _check(lib().pano_batch_set_timing(self._h, int(bool(on))), "set timing")
+
+    def stage_ms(self):
+        ms = np.zeros(4, np.float64)
+        nf = C.c_int()
+        _check(lib().pano_batch_stage_ms(self._h, ms.ctypes.data, C.byref(nf)), "stage times")
+        return dict(zip(self.STAGES, ms.tolist())), nf.value
+
+    def render(self, srcs_per_render, outs, luts_per_render=None, stream=None):
+        """srcs_per_render: list (per render) of lists of BGRx cuda tensors;
+        outs: per render BGRx cuda tensor; luts_per_render: per render (n,3,256) or None."""
+        flat = [t for srcs in srcs_per_render for t in srcs]
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in flat])
+        optrs, osteps = zip(*[_dev_ptr_step(t) for t in outs])
+        keep = []
+        lp = None
+        if luts_per_render is not None:
+            arr = []
+            for lt in luts_per_render:
+                if lt is None:
+                    arr.append(None)
+                else:
+                    a = np.ascontiguousarray(lt, np.uint8)
+                    keep.append(a)
+                    arr.append(a.ctypes.data)
+            lp = (C.c_void_p * len(arr))(*[C.c_void_p(a) if a else None for a in arr])
+        _check(lib().pano_batch_render(self._h, _ptrs(ptrs), _sizes(steps), lp, _ptrs(optrs),
+                                       _sizes(osteps), _stream(stream)), "batch render")
         return True

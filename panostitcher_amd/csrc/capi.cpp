@@ -149,6 +149,32 @@ T* upload_vec(Arena& a, const std::vector<T>& v, int& err)
     return d;
 }
 
+// One launch of the fused 8-bit sequence: a stage over a device segment table.
+struct Launch {
+    int stage, level, nseg, blocks;
+    const Seg* segs;
+};
+
+// Runs a fused sequence on a stream; ev (optional): events recorded before the
+// first launch and after the remap, down, blend and final stages.
+int run_sequence(const std::vector<Launch>& seq, const SrcTable& st, const OutTable& ot,
+                 hipStream_t s, std::vector<hipEvent_t>* ev)
+{
+    if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
+    int prev = kStageRemapDown0;
+    for (const Launch& l : seq) {
+        if (ev && l.stage != prev) {
+            // stage boundaries: remap -> down -> blend -> final (down may be empty)
+            for (int k = prev + 1; k <= l.stage; k++) HIP_TRY(hipEventRecord((*ev)[k], s));
+            prev = l.stage;
+        }
+        HIP_TRY(launch_seg(l.stage, l.segs, l.nseg, l.blocks, l.level, st, ot, s));
+    }
+    if (ev)
+        for (int k = prev + 1; k <= 4; k++) HIP_TRY(hipEventRecord((*ev)[k], s));
+    return PANO_OK;
+}
+
 // Device residency of one prepared rig (blender or render).
 struct DeviceRig {
     Arena arena;
@@ -173,6 +199,8 @@ struct DeviceRig {
     uint8_t* d_luts = nullptr;
     int* d_fin_owners = nullptr;
     uint32_t* d_g0comp = nullptr;           // owner's level-0 value per canvas pixel (8-bit render)
+    Seg* d_segs = nullptr;                  // render() alone: one segment per fused launch
+    std::vector<Launch> seq;                // render() alone: the fused launch sequence
     size_t model_bytes[8] = {};
 
     int levels() const { return prep.levels; }
@@ -221,8 +249,9 @@ struct DeviceRig {
                 pl.add(cells * 8);
                 pl.add(cells);
             }
-            pl.add((size_t)n * 768);
+            pl.add((size_t)n * 768 + 4);
         }
+        pl.add(sizeof(Seg) * (2 * L + 2));
         pl.add(64 << 10);  // slack for per-piece alignment
         int err = arena.reserve(pl.total);
         if (err) return err;
@@ -285,15 +314,6 @@ struct DeviceRig {
         d_planes = upload_vec(arena, planes, err);
         d_planes_user = upload_vec(arena, planes_user, err);
         d_canvas = upload_vec(arena, canvas, err);
-        RenderDesc rd{};
-        rd.label = d_label;
-        rd.label_pitch = prep.cols;
-        rd.full = prep.full;
-        rd.levels = L;
-        rd.plane0 = 0;
-        rd.canvas0 = 0;
-        rd.out = 0;
-        d_rd = upload_vec(arena, std::vector<RenderDesc>{rd}, err);
         if (fused && L > 0) {
             for (DownJob& d : prep.down[0]) {
                 const Geom& g = prep.geom[(size_t)d.src * (L + 1)];
@@ -370,8 +390,62 @@ struct DeviceRig {
                 c.mpitch = mp;
             }
             d_cams = upload_vec(arena, cams, err);
-            d_luts = (uint8_t*)arena.take((size_t)n * 768);
+            // n LUTs, then one byte: whether this frame applies them
+            d_luts = (uint8_t*)arena.take((size_t)n * 768 + 4);
+            if (!d_luts) return PANO_ERR_NOMEM;
+            HIP_TRY(hipMemset(d_luts, 0, (size_t)n * 768 + 4));
             if (err) return err;
+        }
+        RenderDesc rd{};
+        rd.label = d_label;
+        rd.planes = d_planes;
+        rd.canvas = d_canvas;
+        rd.camlist = d_camlist;
+        rd.owners = d_fin_owners;
+        rd.cams = d_cams;
+        rd.luts = d_luts;
+        rd.g0comp = d_g0comp;
+        rd.label_pitch = prep.cols;
+        rd.g0pitch = prep.cols;
+        rd.full = prep.full;
+        rd.levels = L;
+        rd.plane0 = 0;
+        rd.canvas0 = 0;
+        rd.out = 0;
+        rd.ncams = n;
+        d_rd = upload_vec(arena, std::vector<RenderDesc>{rd}, err);
+        if (err) return err;
+        if (fused) {
+            // one segment per launch for render() on its own
+            std::vector<Seg> segs;
+            auto add = [&](const void* jobs, int count) {
+                Seg g{};
+                g.rd = d_rd;
+                g.jobs = jobs;
+                g.begin = 0;
+                g.count = count;
+                segs.push_back(g);
+            };
+            std::vector<Launch> ls;
+            auto launch = [&](int stage, int level, int count) {
+                ls.push_back(Launch{stage, level, 1, count, nullptr});
+            };
+            add(d_down[0], n_down[0]);
+            launch(kStageRemapDown0, 0, n_down[0]);
+            for (int j = 1; j < L; j++) {
+                add(d_down[j], n_down[j]);
+                launch(kStageDown, j, n_down[j]);
+            }
+            for (int j = L; j >= 1; j--) {
+                add(d_blend[j], n_blend[j]);
+                launch(kStageBlend, j, n_blend[j]);
+            }
+            add(d_fin, (int)prep.fin.size());
+            launch(kStageFinal, 0, (int)prep.fin.size());
+            d_segs = upload_vec(arena, segs, err);
+            if (err) return err;
+            for (size_t k = 0; k < ls.size(); k++) ls[k].segs = d_segs + k;
+            seq = ls;
         }
         compute_model_bytes(xmaps != nullptr, src_w, src_h);
         return err;
@@ -419,18 +493,6 @@ struct DeviceRig {
     {
         const int L = prep.levels;
         const Plane* pl = user_kind >= 0 ? d_planes_user : d_planes;
-        if (prep.fused0) {
-            // level 0 -> 1 already ran fused with the remap
-            for (int j = 1; j < L; j++) HIP_TRY(launch_down_u8(d_down[j], n_down[j], pl, s));
-            if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
-            for (int j = L; j >= 1; j--)
-                HIP_TRY(launch_blend_u8(d_blend[j], n_blend[j], d_camlist, pl, d_canvas, d_rd, j, s));
-            if (ev) HIP_TRY(hipEventRecord((*ev)[3], s));
-            HIP_TRY(launch_final_u8(d_fin, (int)prep.fin.size(), d_fin_owners, pl, d_canvas, d_rd,
-                                    d_g0comp, prep.cols, ot, s));
-            if (ev) HIP_TRY(hipEventRecord((*ev)[4], s));
-            return PANO_OK;
-        }
         for (int j = 0; j < L; j++)
             HIP_TRY(launch_down(kind, j == 0 ? user_kind : -1, d_down[j], n_down[j], pl, st, s));
         if (ev) HIP_TRY(hipEventRecord((*ev)[2], s));
@@ -508,6 +570,41 @@ struct pano_render {
     int dst_w = 0, dst_h = 0;
     PinnedRing ring;
     StageTimer timer;
+    int lut_state = 0;                // the device's "apply LUTs" byte
+    std::vector<uint8_t> lut_block;   // n x 768 LUTs + the flag byte
+
+    // Stream-ordered update of the LUT buffer for this frame (fused path).
+    int upload_luts(const uint8_t* luts, hipStream_t s)
+    {
+        const size_t n = rig->prep.n;
+        if (luts) {
+            lut_block.resize(n * 768 + 1);
+            memcpy(lut_block.data(), luts, n * 768);
+            lut_block[n * 768] = 1;
+            lut_state = 1;
+            return ring.upload(lut_block.data(), n * 768 + 1, rig->d_luts, s);
+        }
+        if (lut_state) {
+            static const uint8_t zero = 0;
+            lut_state = 0;
+            return ring.upload(&zero, 1, rig->d_luts + n * 768, s);
+        }
+        return PANO_OK;
+    }
+};
+
+struct pano_batch {
+    std::vector<pano_render*> renders;
+    int total_cams = 0;
+    Seg* d_segs = nullptr;
+    std::vector<Launch> seq;
+    StageTimer timer;
+    hipStream_t last = nullptr;
+    bool used = false;
+    ~pano_batch()
+    {
+        if (d_segs) (void)hipFree(d_segs);
+    }
 };
 
 extern "C" {
@@ -782,13 +879,14 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
     // 8-bit planes hold every Gaussian level exactly when sampled pixels lie
     // inside the masks (as getReprojectMapAndMask produces them); otherwise the
     // reference's pyramid can leave [0, 255] and 16-bit planes are used.
-    r->rig->kind = consistent ? kPixU8 : kPixS16;
+    const bool fused = consistent && true_num_levels(dst_w, dst_h, max_levels, 2) >= 1;
+    r->rig->kind = fused ? kPixU8 : kPixS16;
     int err = prepare_rig(num, dst_h, dst_w, masks, steps.data(), sp.data(), max_levels, 2, false,
-                          r->rig->kind, consistent, r->rig->prep);
+                          r->rig->kind, fused, r->rig->prep);
     if (err) return err;
     err = r->rig->build(true, false, xmaps, ymaps, masks, steps.data(), src_w, src_h);
     if (err) return err;
-    err = r->ring.init((size_t)num * 768);
+    err = r->ring.init((size_t)num * 768 + 1);
     if (err) return err;
     r->src_w.assign(src_w, src_w + num);
     r->src_h.assign(src_h, src_h + num);
@@ -832,29 +930,28 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
     SrcTable st{};
     for (int i = 0; i < num; i++) {
         if (!srcs[i] || src_steps[i] < (size_t)r->src_w[i] * 4 || src_steps[i] % 4 ||
-            src_steps[i] > 0xffffffffu)
+            src_steps[i] * (uint64_t)r->src_h[i] > 0xffffffffu)
             return PANO_ERR_ARG;
         st.p[i] = srcs[i];
         st.step[i] = (uint32_t)src_steps[i];
+    }
+    std::vector<hipEvent_t>* ev = r->timer.begin_frame();
+    OutTable ot{};
+    ot.p[0] = dst;
+    ot.step[0] = (uint32_t)dst_step;
+    if (d.prep.fused0) {
+        int e = r->upload_luts(luts, stream);
+        if (e) return e;
+        return run_sequence(d.seq, st, ot, stream, ev);
     }
     if (luts) {
         int e = r->ring.upload(luts, (size_t)num * 768, d.d_luts, stream);
         if (e) return e;
     }
-    std::vector<hipEvent_t>* ev = r->timer.begin_frame();
     if (ev) HIP_TRY(hipEventRecord((*ev)[0], stream));
-    if (d.prep.fused0) {
-        HIP_TRY(launch_remap_down0(d.d_down.empty() ? nullptr : d.d_down[0],
-                                   d.n_down.empty() ? 0 : d.n_down[0], d.d_cams, d.d_planes,
-                                   d.d_luts, luts != nullptr, st, d.d_g0comp, d.prep.cols, stream));
-    } else {
-        HIP_TRY(launch_remap(d.kind, d.d_remap, (int)d.prep.remap.size(), d.d_cams, d.d_planes,
-                             d.d_luts, luts != nullptr, st, stream));
-    }
+    HIP_TRY(launch_remap(d.kind, d.d_remap, (int)d.prep.remap.size(), d.d_cams, d.d_planes,
+                         d.d_luts, luts != nullptr, st, stream));
     if (ev) HIP_TRY(hipEventRecord((*ev)[1], stream));
-    OutTable ot{};
-    ot.p[0] = dst;
-    ot.step[0] = (uint32_t)dst_step;
     return d.run_pyramid(-1, st, ot, stream, ev);
 }
 
@@ -871,6 +968,8 @@ int pano_render_stage_ms(pano_render* r, double* ms4, int* frames)
     return r->timer.collect(ms4, frames);
 }
 
+int pano_render_fused(const pano_render* r) { return r && r->ok && r->rig->prep.fused0 ? 1 : 0; }
+
 int pano_render_num_images(const pano_render* r) { return r && r->ok ? r->rig->prep.n : 0; }
 int pano_render_num_levels(const pano_render* r) { return r && r->ok ? r->rig->prep.levels : -1; }
 int64_t pano_render_mem_bytes(const pano_render* r)
@@ -881,6 +980,141 @@ int64_t pano_render_model_bytes(const pano_render* r, int stage)
 {
     if (!r || !r->ok || stage < 0 || stage > 4) return -1;
     return (int64_t)r->rig->model_bytes[stage];
+}
+
+// -------------------------------------------------------------------- batch --
+
+int pano_batch_create(pano_batch** out)
+{
+    if (!out) return PANO_ERR_ARG;
+    *out = new (std::nothrow) pano_batch();
+    return *out ? PANO_OK : PANO_ERR_NOMEM;
+}
+
+void pano_batch_destroy(pano_batch* b) { delete b; }
+
+int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
+{
+    if (!b || num <= 0 || num > kMaxBatch || num > kMaxOut || !renders) return PANO_ERR_ARG;
+    int cams = 0, maxL = 0;
+    for (int i = 0; i < num; i++) {
+        pano_render* r = renders[i];
+        if (!r || !r->ok) return PANO_ERR_STATE;
+        if (!r->rig->prep.fused0) return PANO_ERR_UNSUPPORTED;
+        cams += r->rig->prep.n;
+        maxL = std::max(maxL, r->rig->prep.levels);
+    }
+    if (cams > kMaxSrc) return PANO_ERR_UNSUPPORTED;
+    if (b->used) HIP_TRY(hipStreamSynchronize(b->last));
+    // segment tables: one per launch, each render's share of the launch in order
+    std::vector<std::vector<Seg>> tables;
+    std::vector<Launch> ls;
+    auto build = [&](int stage, int level, auto pick) {
+        std::vector<Seg> t;
+        int begin = 0, src_base = 0;
+        for (int i = 0; i < num; i++) {
+            DeviceRig& d = *renders[i]->rig;
+            const void* jobs = nullptr;
+            int count = 0;
+            pick(d, jobs, count);
+            if (count > 0) {
+                Seg g{};
+                g.rd = d.d_rd;
+                g.jobs = jobs;
+                g.begin = begin;
+                g.count = count;
+                g.src_base = src_base;
+                g.out = i;
+                t.push_back(g);
+                begin += count;
+            }
+            src_base += d.prep.n;
+        }
+        if (!t.empty()) {
+            ls.push_back(Launch{stage, level, (int)t.size(), begin, nullptr});
+            tables.push_back(t);
+        }
+    };
+    build(kStageRemapDown0, 0, [&](DeviceRig& d, const void*& j, int& c) {
+        if (d.prep.levels >= 1) { j = d.d_down[0]; c = d.n_down[0]; }
+    });
+    for (int lv = 1; lv < maxL; lv++)
+        build(kStageDown, lv, [&](DeviceRig& d, const void*& j, int& c) {
+            if (lv < d.prep.levels) { j = d.d_down[lv]; c = d.n_down[lv]; }
+        });
+    for (int lv = maxL; lv >= 1; lv--)
+        build(kStageBlend, lv, [&](DeviceRig& d, const void*& j, int& c) {
+            if (lv <= d.prep.levels) { j = d.d_blend[lv]; c = d.n_blend[lv]; }
+        });
+    build(kStageFinal, 0, [&](DeviceRig& d, const void*& j, int& c) {
+        j = d.d_fin;
+        c = (int)d.prep.fin.size();
+    });
+    size_t total = 0;
+    for (auto& t : tables) total += t.size();
+    std::vector<Seg> flat;
+    flat.reserve(total);
+    for (auto& t : tables) flat.insert(flat.end(), t.begin(), t.end());
+    if (b->d_segs) HIP_TRY(hipFree(b->d_segs));
+    b->d_segs = nullptr;
+    if (hipMalloc((void**)&b->d_segs, sizeof(Seg) * flat.size()) != hipSuccess) return PANO_ERR_NOMEM;
+    HIP_TRY(hipMemcpy(b->d_segs, flat.data(), sizeof(Seg) * flat.size(), hipMemcpyHostToDevice));
+    size_t off = 0;
+    for (size_t k = 0; k < ls.size(); k++) {
+        ls[k].segs = b->d_segs + off;
+        off += tables[k].size();
+    }
+    b->seq = ls;
+    b->renders.assign(renders, renders + num);
+    b->total_cams = cams;
+    return PANO_OK;
+}
+
+int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                      const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                      hipStream_t stream)
+{
+    if (!b) return PANO_ERR_ARG;
+    if (b->renders.empty()) return PANO_ERR_STATE;
+    if (!srcs || !src_steps || !dsts || !dst_steps) return PANO_ERR_ARG;
+    SrcTable st{};
+    OutTable ot{};
+    int k = 0;
+    for (size_t i = 0; i < b->renders.size(); i++) {
+        pano_render* r = b->renders[i];
+        if (!dsts[i] || dst_steps[i] < (size_t)r->dst_w * 4 || dst_steps[i] % 4 ||
+            dst_steps[i] > 0xffffffffu)
+            return PANO_ERR_ARG;
+        ot.p[i] = dsts[i];
+        ot.step[i] = (uint32_t)dst_steps[i];
+        for (int c = 0; c < r->rig->prep.n; c++, k++) {
+            if (!srcs[k] || src_steps[k] < (size_t)r->src_w[c] * 4 || src_steps[k] % 4 ||
+                src_steps[k] * (uint64_t)r->src_h[c] > 0xffffffffu)
+                return PANO_ERR_ARG;
+            st.p[k] = srcs[k];
+            st.step[k] = (uint32_t)src_steps[k];
+        }
+    }
+    for (size_t i = 0; i < b->renders.size(); i++) {
+        int e = b->renders[i]->upload_luts(luts ? luts[i] : nullptr, stream);
+        if (e) return e;
+    }
+    b->last = stream;
+    b->used = true;
+    return run_sequence(b->seq, st, ot, stream, b->timer.begin_frame());
+}
+
+int pano_batch_set_timing(pano_batch* b, int enable)
+{
+    if (!b) return PANO_ERR_ARG;
+    b->timer.on = enable != 0;
+    return PANO_OK;
+}
+
+int pano_batch_stage_ms(pano_batch* b, double* ms4, int* frames)
+{
+    if (!b || !ms4 || !frames) return PANO_ERR_ARG;
+    return b->timer.collect(ms4, frames);
 }
 
 }  // extern "C"

@@ -128,27 +128,35 @@ __device__ __forceinline__ int lane(uint32_t v, int c) { return (int)((v >> (8 *
 
 }  // namespace
 
+// Segment of a batched launch that owns block b (segments are few and ordered).
+__device__ __forceinline__ const Seg& seg_of(const Seg* segs, int nseg, int b)
+{
+    int s = 0;
+    while (s + 1 < nseg && b >= segs[s + 1].begin) s++;
+    return segs[s];
+}
+
 // ------------------------------------------------------------ level 0 -> 1 --
 
-template <bool kLut>
-__global__ __launch_bounds__(kDT) void k_remap_down0(const DownJob* __restrict__ jobs,
-                                                     const RemapCam* __restrict__ cams,
-                                                     const Plane* __restrict__ planes,
-                                                     const uint8_t* __restrict__ luts, SrcTable src,
-                                                     uint32_t* __restrict__ g0comp, int g0pitch)
+__global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ segs, int nseg,
+                                                     SrcTable src)
 {
     __shared__ uint32_t in[kIH * kIW];
     __shared__ uint2 hs[kIH * kFTW];
     __shared__ uint8_t lut_s[768];
-    const DownJob J = jobs[blockIdx.x];
-    const RemapCam cam = cams[J.src];
-    const bool use_lut = kLut && cam.lut >= 0;
+    const Seg& S = seg_of(segs, nseg, blockIdx.x);
+    const RenderDesc R = ldc(S.rd);
+    const DownJob J = ldc((const DownJob*)S.jobs + (blockIdx.x - S.begin));
+    const RemapCam cam = ldc(R.cams + J.src);
+    const bool use_lut = cam.lut >= 0 && (ldc((const uint32_t*)(R.luts + R.ncams * 768)) & 255u);
     if (use_lut)
-        for (int e = threadIdx.x; e < 768; e += kDT) lut_s[e] = luts[cam.lut + e];
-    const Plane P0 = planes[cam.plane];
-    const Plane D = planes[J.dst];
-    const uint8_t* sp = src.p[cam.src];
-    const uint32_t sstep = src.step[cam.src];
+        for (int e = threadIdx.x; e < 768; e += kDT) lut_s[e] = ldg(R.luts + cam.lut + e);
+    const Plane P0 = ldc(R.planes + cam.plane);
+    const Plane D = ldc(R.planes + J.dst);
+    const uint8_t* sp = src.p[S.src_base + cam.src];
+    const uint32_t sstep = src.step[S.src_base + cam.src];
+    uint32_t* g0comp = R.g0comp;
+    const int g0pitch = R.g0pitch;
     const int bx0 = 2 * J.X - 2, by0 = 2 * J.Y - 2;
     const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
     // Software pipeline over kRounds rounds of kBatch pixels per thread: the
@@ -209,14 +217,15 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const DownJob* __restrict__
 
 // ------------------------------------------------------- level j -> j+1 --
 
-__global__ __launch_bounds__(kDT) void k_down_u8(const DownJob* __restrict__ jobs,
-                                                 const Plane* __restrict__ planes)
+__global__ __launch_bounds__(kDT) void k_down_u8(const Seg* __restrict__ segs, int nseg)
 {
     __shared__ uint32_t in[kIH * kIW];
     __shared__ uint2 hs[kIH * kFTW];
-    const DownJob J = jobs[blockIdx.x];
-    const Plane S = planes[J.src];
-    const Plane D = planes[J.dst];
+    const Seg& Sg = seg_of(segs, nseg, blockIdx.x);
+    const DownJob J = ldc((const DownJob*)Sg.jobs + (blockIdx.x - Sg.begin));
+    const Plane* planes = ldc(&Sg.rd->planes);
+    const Plane S = ldc(planes + J.src);
+    const Plane D = ldc(planes + J.dst);
     const int bx0 = 2 * J.X - 2, by0 = 2 * J.Y - 2;
     const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
     uint32_t v[kPer];
@@ -240,22 +249,22 @@ __global__ __launch_bounds__(kDT) void k_down_u8(const DownJob* __restrict__ job
 // ------------------------------------------------------- blend, level >= 1 --
 
 constexpr int kSW = kQTW + 2, kSH = kQTH + 2;  // staged level-(j+1) region
+constexpr int kStageIt = (kSH * kSW + 255) / 256;
 
-__global__ __launch_bounds__(256) void k_blend_u8(const TileJob* __restrict__ jobs,
-                                                  const int* __restrict__ camlist,
-                                                  const Plane* __restrict__ planes,
-                                                  const CanvasLevel* __restrict__ canvas,
-                                                  const RenderDesc* __restrict__ rds, int level)
+__global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, int nseg, int level)
 {
     __shared__ uint32_t gs[kMaxStage][kSH * kSW];
     __shared__ int4 cs[kSH * kSW];
-    const TileJob J = jobs[blockIdx.x];
-    const RenderDesc R = rds[J.r];
+    const Seg& Sg = seg_of(segs, nseg, blockIdx.x);
+    const RenderDesc R = ldc(Sg.rd);
+    const TileJob J = ldc((const TileJob*)Sg.jobs + (blockIdx.x - Sg.begin));
+    const Plane* planes = R.planes;
+    const int* camlist = R.camlist;
     const bool top = J.flags & kTileTop;
     const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
     const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
     const int sx0 = (J.X >> 1) - 1, sy0 = (J.Y >> 1) - 1;
-    const CanvasLevel C = canvas[R.canvas0 + level];
+    const CanvasLevel C = ldc(R.canvas + level);
     bool in[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) in[d] = X + (d & 1) < J.X + J.nx && Y + (d >> 1) < J.Y + J.ny;
@@ -265,9 +274,8 @@ __global__ __launch_bounds__(256) void k_blend_u8(const TileJob* __restrict__ jo
         for (int d = 0; d < 4; d++)
             if (in[d]) ws[d] = ldg(C.wsum + (Y + (d >> 1)) * C.pitch + X + (d & 1));
     // stage up to kMaxStage cameras' level j+1 regions and C_{j+1} with every load in flight
-    constexpr int kStageIt = (kSH * kSW + 255) / 256;
     if (!top) {
-        const CanvasLevel C1 = canvas[R.canvas0 + level + 1];
+        const CanvasLevel C1 = ldc(R.canvas + level + 1);
         int4 cv[kStageIt];
 #pragma unroll
         for (int k = 0; k < kStageIt; k++) {
@@ -292,7 +300,7 @@ __global__ __launch_bounds__(256) void k_blend_u8(const TileJob* __restrict__ jo
             uint32_t gv[kMaxStage][kStageIt];
 #pragma unroll
             for (int s = 0; s < kMaxStage; s++) {
-                const Plane P1 = planes[camlist[J.off + g + (s < ng ? s : 0)] + 1];
+                const Plane P1 = ldc(planes + ldc(camlist + J.off + g + (s < ng ? s : 0)) + 1);
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++) {
                     int e = threadIdx.x + k * 256;
@@ -312,7 +320,7 @@ __global__ __launch_bounds__(256) void k_blend_u8(const TileJob* __restrict__ jo
         }
         __syncthreads();
         for (int s = 0; s < ng; s++) {
-            const Plane P = planes[camlist[J.off + g + s]];
+            const Plane P = ldc(planes + ldc(camlist + J.off + g + s));
             int idx[4], w[4];
             uint32_t gp[4];
 #pragma unroll
@@ -360,23 +368,20 @@ __global__ __launch_bounds__(256) void k_blend_u8(const TileJob* __restrict__ jo
 
 // ------------------------------------------------------------ level 0 --
 
-__global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jobs,
-                                                  const int* __restrict__ owners,
-                                                  const Plane* __restrict__ planes,
-                                                  const CanvasLevel* __restrict__ canvas,
-                                                  const RenderDesc* __restrict__ rds,
-                                                  const uint32_t* __restrict__ g0comp,
-                                                  int g0pitch, OutTable outs)
+__global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, int nseg,
+                                                  OutTable outs)
 {
     __shared__ uint32_t gs[kMaxStage][kSH * kSW];
     __shared__ int4 cs[kSH * kSW];
-    const TileJob J = jobs[blockIdx.x];
-    const RenderDesc R = rds[J.r];
+    const Seg& Sg = seg_of(segs, nseg, blockIdx.x);
+    const RenderDesc R = ldc(Sg.rd);
+    const TileJob J = ldc((const TileJob*)Sg.jobs + (blockIdx.x - Sg.begin));
+    const Plane* planes = R.planes;
+    const int* owners = R.owners;
     const int L = R.levels;
     const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
     const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
     const int sx0 = (J.X >> 1) - 1, sy0 = (J.Y >> 1) - 1;
-    constexpr int kStageIt = (kSH * kSW + 255) / 256;
     bool in[4];
     int lab[4];
     uint32_t g0[4];
@@ -385,10 +390,10 @@ __global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jo
         int Xp = X + (d & 1), Yp = Y + (d >> 1);
         in[d] = Xp < J.X + J.nx && Yp < J.Y + J.ny;
         lab[d] = in[d] ? (int)ldg(R.label + Yp * R.label_pitch + Xp) : (int)kNoOwner;
-        g0[d] = in[d] ? g0comp[(size_t)Yp * g0pitch + Xp] : 0u;
+        g0[d] = in[d] ? ldg(R.g0comp + (size_t)Yp * R.g0pitch + Xp) : 0u;
     }
     if (L > 0) {
-        const CanvasLevel C1 = canvas[R.canvas0 + 1];
+        const CanvasLevel C1 = ldc(R.canvas + 1);
         int4 cv[kStageIt];
 #pragma unroll
         for (int k = 0; k < kStageIt; k++) {
@@ -418,8 +423,8 @@ __global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jo
             uint32_t gv[kMaxStage][kStageIt];
 #pragma unroll
             for (int s = 0; s < kMaxStage; s++) {
-                const int o = owners[J.off + g + (s < ng ? s : 0)];
-                const Plane P1 = planes[R.plane0 + o * (L + 1) + 1];
+                const int o = ldc(owners + J.off + g + (s < ng ? s : 0));
+                const Plane P1 = ldc(planes + o * (L + 1) + 1);
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++) {
                     int e = threadIdx.x + k * 256;
@@ -438,7 +443,7 @@ __global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jo
                 }
             __syncthreads();
             for (int s = 0; s < ng; s++) {
-                const int o = owners[J.off + g + s];
+                const int o = ldc(owners + J.off + g + s);
                 if (lab[0] != o && lab[1] != o && lab[2] != o && lab[3] != o) continue;
                 uint32_t u[4];
                 up_quad_u8(&gs[s][(qy + 1) * kSW + qx + 1], kSW, u);
@@ -453,8 +458,8 @@ __global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jo
         if (J.cnt == 0) __syncthreads();  // cs staged, no owner loop ran
         up_quad_i32(cs + (qy + 1) * kSW + qx + 1, kSW, up);
     }
-    uint8_t* out = outs.p[R.out];
-    const uint32_t ostep = outs.step[R.out];
+    uint8_t* out = outs.p[Sg.out];
+    const uint32_t ostep = outs.step[Sg.out];
 #pragma unroll
     for (int d = 0; d < 4; d++) {
         if (!in[d]) continue;
@@ -469,44 +474,26 @@ __global__ __launch_bounds__(256) void k_final_u8(const TileJob* __restrict__ jo
 
 // ------------------------------------------------------------- launchers --
 
-hipError_t launch_remap_down0(const DownJob* jobs, int njobs, const RemapCam* cams,
-                              const Plane* planes, const uint8_t* luts, bool use_lut,
-                              const SrcTable& src, uint32_t* g0comp, int g0pitch, hipStream_t s)
+hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int level,
+                      const SrcTable& src, const OutTable& outs, hipStream_t s)
 {
-    if (njobs <= 0) return hipSuccess;
-    if (use_lut)
-        hipLaunchKernelGGL(k_remap_down0<true>, dim3(njobs), dim3(kDT), 0, s, jobs, cams, planes,
-                           luts, src, g0comp, g0pitch);
-    else
-        hipLaunchKernelGGL(k_remap_down0<false>, dim3(njobs), dim3(kDT), 0, s, jobs, cams, planes,
-                           luts, src, g0comp, g0pitch);
-    return hipGetLastError();
-}
-
-hipError_t launch_down_u8(const DownJob* jobs, int njobs, const Plane* planes, hipStream_t s)
-{
-    if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_down_u8, dim3(njobs), dim3(kDT), 0, s, jobs, planes);
-    return hipGetLastError();
-}
-
-hipError_t launch_blend_u8(const TileJob* jobs, int njobs, const int* camlist, const Plane* planes,
-                           const CanvasLevel* canvas, const RenderDesc* rd, int level,
-                           hipStream_t s)
-{
-    if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_blend_u8, dim3(njobs), dim3(256), 0, s, jobs, camlist, planes, canvas, rd,
-                       level);
-    return hipGetLastError();
-}
-
-hipError_t launch_final_u8(const TileJob* jobs, int njobs, const int* owners, const Plane* planes,
-                           const CanvasLevel* canvas, const RenderDesc* rd, const uint32_t* g0comp,
-                           int g0pitch, const OutTable& outs, hipStream_t s)
-{
-    if (njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_final_u8, dim3(njobs), dim3(256), 0, s, jobs, owners, planes, canvas, rd,
-                       g0comp, g0pitch, outs);
+    if (blocks <= 0 || nseg <= 0) return hipSuccess;
+    switch (stage) {
+    case kStageRemapDown0:
+        hipLaunchKernelGGL(k_remap_down0, dim3(blocks), dim3(kDT), 0, s, segs, nseg, src);
+        break;
+    case kStageDown:
+        hipLaunchKernelGGL(k_down_u8, dim3(blocks), dim3(kDT), 0, s, segs, nseg);
+        break;
+    case kStageBlend:
+        hipLaunchKernelGGL(k_blend_u8, dim3(blocks), dim3(256), 0, s, segs, nseg, level);
+        break;
+    case kStageFinal:
+        hipLaunchKernelGGL(k_final_u8, dim3(blocks), dim3(256), 0, s, segs, nseg, outs);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -60,15 +60,40 @@ struct CanvasLevel {
     int pad;
 };
 
-struct RenderDesc {   // per render (rig) of a launch
-    const uint8_t* label;  // level-0 owner per canvas pixel
+struct RemapCam;
+
+// Per render (rig), in device memory.  The generic 16-bit kernels index the
+// launch's arrays with plane0 / canvas0 / out; the 8-bit kernels reach every
+// array of their render through the pointers.
+struct RenderDesc {
+    const uint8_t* label;        // level-0 owner per canvas pixel
+    const Plane* planes;         // n * (levels + 1)
+    const CanvasLevel* canvas;   // levels + 1
+    const int* camlist;          // blend cameras (plane indices)
+    const int* owners;           // level-0 owner cameras
+    const RemapCam* cams;        // remap inputs
+    const uint8_t* luts;         // n x 768 LUT bytes, then 1 byte: LUTs active this frame
+    uint32_t* g0comp;            // owners' level-0 samples
     int label_pitch;
-    int full;              // masks cover the whole canvas
+    int g0pitch;
+    int full;                    // masks cover the whole canvas
     int levels;
-    int plane0;            // index of (camera 0, level 0) in the plane array
-    int canvas0;           // index of level 0 in the canvas array
-    int out;               // OutTable index of its output canvas
+    int plane0;                  // (generic kernels) index of (camera 0, level 0)
+    int canvas0;                 // (generic kernels) index of level 0
+    int out;                     // (generic kernels) OutTable index
+    int ncams;
 };
+
+// One render's share of a batched launch: blocks [begin, begin + count) run its
+// jobs; its cameras' frames start at SrcTable index src_base; its canvas is
+// OutTable entry out.
+struct Seg {
+    const RenderDesc* rd;
+    const void* jobs;
+    int begin, count, src_base, out;
+};
+
+constexpr int kMaxBatch = 32;   // renders per batched launch
 
 // Per camera of the render path (resident).  map/mask cover the camera's
 // level-0 box.  For the 8-bit fused path they carry a kMapMargin frame of

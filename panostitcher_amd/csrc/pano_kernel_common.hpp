@@ -23,6 +23,22 @@ __device__ __forceinline__ void stg(T* p, T v)
 {
     *(PANO_GLOBAL T*)p = v;
 }
+// Block-uniform descriptor reads (jobs, planes, canvas levels, camera lists)
+// through pointers held in other descriptors: read word by word from the
+// constant address space so that they become scalar loads.
+#define PANO_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ T ldc(const T* p)
+{
+    static_assert(sizeof(T) % 4 == 0, "descriptor size must be a multiple of 4");
+    T v;
+    uint32_t* d = (uint32_t*)&v;
+    const PANO_CONST uint32_t* s = (const PANO_CONST uint32_t*)p;
+#pragma unroll
+    for (unsigned i = 0; i < sizeof(T) / 4; i++) d[i] = s[i];
+    return v;
+}
+
 // HIP's vector structs do not take the address-space cast as one access; go
 // through the compiler's native vector types.
 typedef int pano_i32x4 __attribute__((ext_vector_type(4)));

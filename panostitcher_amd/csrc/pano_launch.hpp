@@ -32,17 +32,11 @@ hipError_t launch_reproject16s(const uint8_t* src, size_t src_step, int sw, int 
 hipError_t launch_transform(const uint8_t* src, size_t src_step, uint8_t* dst, size_t dst_step,
                             int w, int h, const uint8_t* lut_dev, hipStream_t s);
 
-// Fused 8-bit path (kernels_u8.hip).
-hipError_t launch_remap_down0(const DownJob* jobs, int njobs, const RemapCam* cams,
-                              const Plane* planes, const uint8_t* luts, bool use_lut,
-                              const SrcTable& src, uint32_t* g0comp, int g0pitch, hipStream_t s);
-hipError_t launch_down_u8(const DownJob* jobs, int njobs, const Plane* planes, hipStream_t s);
-hipError_t launch_blend_u8(const TileJob* jobs, int njobs, const int* camlist, const Plane* planes,
-                           const CanvasLevel* canvas, const RenderDesc* rd, int level,
-                           hipStream_t s);
-hipError_t launch_final_u8(const TileJob* jobs, int njobs, const int* owners, const Plane* planes,
-                           const CanvasLevel* canvas, const RenderDesc* rd, const uint32_t* g0comp,
-                           int g0pitch, const OutTable& outs, hipStream_t s);
+// Fused 8-bit path (kernels_u8.hip): one launch of a stage over the segments
+// of a batch (blocks = sum of the segments' job counts).
+enum Stage : int { kStageRemapDown0 = 0, kStageDown = 1, kStageBlend = 2, kStageFinal = 3 };
+hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int level,
+                      const SrcTable& src, const OutTable& outs, hipStream_t s);
 
 // Zero-fill with a kernel (stays correct inside captured graphs).
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);

@@ -187,3 +187,34 @@ def test_render_inconsistent_maps_use_16bit_planes():
     out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
     assert r.render([to_dev(synth.bgrx(s)) for s in srcs], out, luts)
     assert_parity(from_dev(out)[..., :3], ref, exact=True, label="inconsistent")
+
+
+def test_batch_matches_alone_and_oracle():
+    # Renders of different rigs and sizes stitched in one launch per stage give
+    # the same canvases as each render alone, and the oracle's.
+    cases = [("fisheye2", 0.25), ("rect6", 0.25), ("mixed8", 0.125)]
+    renders, srcs, luts, refs, alone = [], [], [], [], []
+    for name, scale in cases:
+        wl = synth.workload(name, scale)
+        got, ref, r, dsrc, lt = run_render(wl)
+        assert r.fused
+        renders.append(r)
+        srcs.append(dsrc)
+        luts.append(lt)
+        refs.append(ref)
+        alone.append(got)
+    note("batch of 3 renders")
+    b = P.Batch(renders)
+    outs = [torch.zeros_like(torch.from_numpy(a)).cuda() for a in alone]
+    for _ in range(2):
+        b.render(srcs, outs, luts)
+    for k, o in enumerate(outs):
+        got = from_dev(o)
+        np.testing.assert_array_equal(got, alone[k])
+        assert_parity(got[..., :3], refs[k], exact=True, label=f"batch[{k}]")
+    # a different composition and order
+    b.set([renders[2], renders[0]])
+    o2 = [torch.zeros_like(outs[2]), torch.zeros_like(outs[0])]
+    b.render([srcs[2], srcs[0]], o2, [luts[2], luts[0]])
+    np.testing.assert_array_equal(from_dev(o2[0]), alone[2])
+    np.testing.assert_array_equal(from_dev(o2[1]), alone[0])
