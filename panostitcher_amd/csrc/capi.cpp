@@ -138,6 +138,27 @@ int host_reflect(int p, int n)
 int map_pitch(const Geom& g) { return g.bw > 0 ? (g.bw + 2 * kMapMargin + 63) / 64 * 64 : 0; }
 size_t map_cells(const Geom& g) { return g.bh > 0 ? (size_t)map_pitch(g) * (g.bh + 2 * kMapMargin) : 0; }
 
+// Row step of a LUT-mapped frame copy.
+size_t lut_step(int w) { return ((size_t)w * 4 + 255) / 256 * 256; }
+
+// A map point in fused storage: |x|, |y| with the flags in the sign bits, or
+// +inf for a point the remap treats as outside the source (including NaN).
+float2 fused_point(float x, float y, int sw, int sh, uint8_t flags)
+{
+    const bool inside = x >= 0.f && y >= 0.f && x < (float)sw && y < (float)sh;
+    uint32_t bx, by;
+    memcpy(&bx, &x, 4);
+    memcpy(&by, &y, 4);
+    bx = inside ? bx & 0x7fffffffu : 0x7f800000u;
+    by = inside ? by & 0x7fffffffu : 0x7f800000u;
+    if (flags & kFlagOwned) bx |= 0x80000000u;
+    if (flags & kFlagValid) by |= 0x80000000u;
+    float2 r;
+    memcpy(&r.x, &bx, 4);
+    memcpy(&r.y, &by, 4);
+    return r;
+}
+
 template <class T>
 T* upload_vec(Arena& a, const std::vector<T>& v, int& err)
 {
@@ -155,12 +176,21 @@ struct Launch {
     const Seg* segs;
 };
 
+// The gain LUT pass ahead of a fused sequence: jobs over the frames of src.
+struct LutPass {
+    const LutJob* jobs = nullptr;
+    int njobs = 0, blocks = 0;
+    SrcTable src{};
+};
+
 // Runs a fused sequence on a stream; ev (optional): events recorded before the
-// first launch and after the remap, down, blend and final stages.
+// first launch and after the remap (with the LUT pass), down, blend and final
+// stages.  st: the frames the remap reads (LUT-mapped copies where lut is given).
 int run_sequence(const std::vector<Launch>& seq, const SrcTable& st, const OutTable& ot,
-                 hipStream_t s, std::vector<hipEvent_t>* ev)
+                 hipStream_t s, std::vector<hipEvent_t>* ev, const LutPass* lut = nullptr)
 {
     if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
+    if (lut) HIP_TRY(launch_lut(lut->jobs, lut->njobs, lut->blocks, lut->src, s));
     int prev = kStageRemapDown0;
     for (const Launch& l : seq) {
         if (ev && l.stage != prev) {
@@ -201,6 +231,11 @@ struct DeviceRig {
     uint32_t* d_g0comp = nullptr;           // owner's level-0 value per canvas pixel (8-bit render)
     Seg* d_segs = nullptr;                  // render() alone: one segment per fused launch
     std::vector<Launch> seq;                // render() alone: the fused launch sequence
+    std::vector<uint8_t*> lsrc;             // fused: LUT-mapped copy of each camera's frame
+    std::vector<uint32_t> lstep;
+    std::vector<LutJob> lutjobs;            // fused: LUT pass over this render's frames
+    LutJob* d_lutjobs = nullptr;
+    int lut_blocks = 0;
     size_t model_bytes[8] = {};
 
     int levels() const { return prep.levels; }
@@ -247,8 +282,10 @@ struct DeviceRig {
                 const Geom& g = prep.geom[(size_t)i * (L + 1)];
                 size_t cells = fused ? map_cells(g) : g.pixels();
                 pl.add(cells * 8);
-                pl.add(cells);
+                if (!fused) pl.add(cells);
+                if (fused) pl.add(lut_step(src_w[i]) * (size_t)src_h[i]);
             }
+            if (fused) pl.add(sizeof(LutJob) * n);
             pl.add((size_t)n * 768 + 4);
         }
         pl.add(sizeof(Seg) * (2 * L + 2));
@@ -364,9 +401,10 @@ struct DeviceRig {
                 const int mrows = g.bh > 0 ? g.bh + 2 * m : 0;
                 const size_t cells = (size_t)mp * mrows;
                 float2* dm = (float2*)arena.take(cells * 8);
-                uint8_t* dk = (uint8_t*)arena.take(cells);
-                std::vector<float2> hm(cells, make_float2(-1.f, -1.f));
-                std::vector<uint8_t> hk(cells, 0);
+                uint8_t* dk = fused ? nullptr : (uint8_t*)arena.take(cells);
+                if (!dm || (!fused && !dk && cells)) return PANO_ERR_NOMEM;
+                std::vector<float2> hm(cells, fused ? fused_point(-1.f, -1.f, 1, 1, 0) : make_float2(-1.f, -1.f));
+                std::vector<uint8_t> hk(fused ? 0 : cells, 0);
                 // entry (sy, sx) holds canvas (x0 - m + sx, y0 - m + sy) after border
                 // resolution, or nothing when that lies outside the box
                 for (int sy = 0; sy < mrows; sy++)
@@ -376,14 +414,19 @@ struct DeviceRig {
                         if (lx < 0) lx += g.cw;
                         if (ly < 0 || ly >= g.bh || lx >= g.bw) continue;
                         size_t o = (size_t)Y * prep.cols + X;
-                        hm[(size_t)sy * mp + sx] = make_float2(xmaps[i][o], ymaps[i][o]);
                         uint8_t f = masks[i][(size_t)Y * mstep[i] + X] ? kFlagValid : 0;
                         if (prep.label[o] == i) f |= kFlagOwned;
-                        hk[(size_t)sy * mp + sx] = f;
+                        if (fused) {
+                            hm[(size_t)sy * mp + sx] =
+                                fused_point(xmaps[i][o], ymaps[i][o], src_w[i], src_h[i], f);
+                        } else {
+                            hm[(size_t)sy * mp + sx] = make_float2(xmaps[i][o], ymaps[i][o]);
+                            hk[(size_t)sy * mp + sx] = f;
+                        }
                     }
                 if (cells) {
                     HIP_TRY(hipMemcpy(dm, hm.data(), cells * 8, hipMemcpyHostToDevice));
-                    HIP_TRY(hipMemcpy(dk, hk.data(), cells, hipMemcpyHostToDevice));
+                    if (dk) HIP_TRY(hipMemcpy(dk, hk.data(), cells, hipMemcpyHostToDevice));
                 }
                 c.map = dm;
                 c.mask = dk;
@@ -394,6 +437,30 @@ struct DeviceRig {
             d_luts = (uint8_t*)arena.take((size_t)n * 768 + 4);
             if (!d_luts) return PANO_ERR_NOMEM;
             HIP_TRY(hipMemset(d_luts, 0, (size_t)n * 768 + 4));
+            if (fused) {
+                lsrc.assign(n, nullptr);
+                lstep.assign(n, 0);
+                lutjobs.clear();
+                lut_blocks = 0;
+                for (int i = 0; i < n; i++) {
+                    lstep[i] = (uint32_t)lut_step(src_w[i]);
+                    lsrc[i] = (uint8_t*)arena.take((size_t)lstep[i] * src_h[i]);
+                    if (!lsrc[i]) return PANO_ERR_NOMEM;
+                    LutJob J{};
+                    J.lut = d_luts + (size_t)i * 768;
+                    J.flag = d_luts + (size_t)n * 768;
+                    J.out = lsrc[i];
+                    J.out_step = lstep[i];
+                    J.src = i;
+                    J.w = src_w[i];
+                    J.h = src_h[i];
+                    J.rows = std::max(1, 8192 / src_w[i]);
+                    J.begin = lut_blocks;
+                    lut_blocks += (J.h + J.rows - 1) / J.rows;
+                    lutjobs.push_back(J);
+                }
+                d_lutjobs = upload_vec(arena, lutjobs, err);
+            }
             if (err) return err;
         }
         RenderDesc rd{};
@@ -466,7 +533,7 @@ struct DeviceRig {
         for (int j = 0; j < L; j++)
             for (const DownJob& d : prep.down[j]) {
                 size_t outp = (size_t)d.nx * d.ny, inp = 4 * outp;
-                if (j == 0 && prep.fused0) s0 += inp * (8 + 1) + outp * px;
+                if (j == 0 && prep.fused0) s0 += inp * 8 + outp * px;
                 else if (j == 0 && fused_maps) s0 += inp * (8 + 1 + px) + inp * px + outp * px;
                 else s1 += inp * px + outp * px;
             }
@@ -597,6 +664,8 @@ struct pano_batch {
     std::vector<pano_render*> renders;
     int total_cams = 0;
     Seg* d_segs = nullptr;
+    LutJob* d_lutjobs = nullptr;        // every render's LUT jobs, SrcTable indices of the batch
+    int n_lutjobs = 0, lut_blocks = 0;
     std::vector<Launch> seq;
     StageTimer timer;
     hipStream_t last = nullptr;
@@ -604,6 +673,7 @@ struct pano_batch {
     ~pano_batch()
     {
         if (d_segs) (void)hipFree(d_segs);
+        if (d_lutjobs) (void)hipFree(d_lutjobs);
     }
 };
 
@@ -859,6 +929,7 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
     if ((dst_w & 1) || (dst_h & 1) || dst_h * 2 != dst_w || dst_w <= 0) return PANO_ERR_ARG;
     if (max_levels <= 0) return PANO_ERR_ARG;
     const size_t P = (size_t)dst_w * dst_h;
+    if (P * 4 > 0xffffffffu) return PANO_ERR_UNSUPPORTED;   // 32-bit canvas offsets
     std::vector<size_t> steps(num, (size_t)dst_w);
     // pixels whose level-0 value can be non-zero: mask or a valid map point
     std::vector<std::vector<uint8_t>> support(num, std::vector<uint8_t>(P));
@@ -866,6 +937,9 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
     bool consistent = true;
     for (int i = 0; i < num; i++) {
         if (!xmaps[i] || !ymaps[i] || !masks[i] || src_w[i] <= 0 || src_h[i] <= 0) return PANO_ERR_ARG;
+        // 24-bit row offsets in the remap, 32-bit frame offsets
+        if (lut_step(src_w[i]) >= (1u << 24) || lut_step(src_w[i]) * (uint64_t)src_h[i] > 0xffffffffu)
+            return PANO_ERR_UNSUPPORTED;
         for (size_t p = 0; p < P; p++) {
             double x = xmaps[i][p], y = ymaps[i][p];
             bool v = x >= 0 && y >= 0 && x < src_w[i] && y < src_h[i];
@@ -930,7 +1004,7 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
     SrcTable st{};
     for (int i = 0; i < num; i++) {
         if (!srcs[i] || src_steps[i] < (size_t)r->src_w[i] * 4 || src_steps[i] % 4 ||
-            src_steps[i] * (uint64_t)r->src_h[i] > 0xffffffffu)
+            src_steps[i] * (uint64_t)r->src_h[i] > 0xffffffffu || src_steps[i] >= (1u << 24))
             return PANO_ERR_ARG;
         st.p[i] = srcs[i];
         st.step[i] = (uint32_t)src_steps[i];
@@ -942,7 +1016,18 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
     if (d.prep.fused0) {
         int e = r->upload_luts(luts, stream);
         if (e) return e;
-        return run_sequence(d.seq, st, ot, stream, ev);
+        if (!luts) return run_sequence(d.seq, st, ot, stream, ev);
+        LutPass lp;
+        lp.jobs = d.d_lutjobs;
+        lp.njobs = (int)d.lutjobs.size();
+        lp.blocks = d.lut_blocks;
+        lp.src = st;
+        SrcTable sr = st;
+        for (int i = 0; i < num; i++) {
+            sr.p[i] = d.lsrc[i];
+            sr.step[i] = d.lstep[i];
+        }
+        return run_sequence(d.seq, sr, ot, stream, ev, &lp);
     }
     if (luts) {
         int e = r->ring.upload(luts, (size_t)num * 768, d.d_luts, stream);
@@ -1064,6 +1149,25 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
         ls[k].segs = b->d_segs + off;
         off += tables[k].size();
     }
+    std::vector<LutJob> lj;
+    int lblocks = 0, base = 0;
+    for (int i = 0; i < num; i++) {
+        const DeviceRig& d = *renders[i]->rig;
+        for (LutJob J : d.lutjobs) {
+            J.src += base;
+            J.begin = lblocks;
+            lblocks += (J.h + J.rows - 1) / J.rows;
+            lj.push_back(J);
+        }
+        base += d.prep.n;
+    }
+    if (b->d_lutjobs) HIP_TRY(hipFree(b->d_lutjobs));
+    b->d_lutjobs = nullptr;
+    if (hipMalloc((void**)&b->d_lutjobs, sizeof(LutJob) * lj.size()) != hipSuccess)
+        return PANO_ERR_NOMEM;
+    HIP_TRY(hipMemcpy(b->d_lutjobs, lj.data(), sizeof(LutJob) * lj.size(), hipMemcpyHostToDevice));
+    b->n_lutjobs = (int)lj.size();
+    b->lut_blocks = lblocks;
     b->seq = ls;
     b->renders.assign(renders, renders + num);
     b->total_cams = cams;
@@ -1089,19 +1193,36 @@ int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* s
         ot.step[i] = (uint32_t)dst_steps[i];
         for (int c = 0; c < r->rig->prep.n; c++, k++) {
             if (!srcs[k] || src_steps[k] < (size_t)r->src_w[c] * 4 || src_steps[k] % 4 ||
-                src_steps[k] * (uint64_t)r->src_h[c] > 0xffffffffu)
+                src_steps[k] * (uint64_t)r->src_h[c] > 0xffffffffu || src_steps[k] >= (1u << 24))
                 return PANO_ERR_ARG;
             st.p[k] = srcs[k];
             st.step[k] = (uint32_t)src_steps[k];
         }
     }
+    // frames of renders with LUTs this call are read from their LUT-mapped copies
+    SrcTable sr = st;
+    bool any = false;
+    k = 0;
     for (size_t i = 0; i < b->renders.size(); i++) {
-        int e = b->renders[i]->upload_luts(luts ? luts[i] : nullptr, stream);
+        pano_render* r = b->renders[i];
+        const uint8_t* lt = luts ? luts[i] : nullptr;
+        int e = r->upload_luts(lt, stream);
         if (e) return e;
+        for (int c = 0; c < r->rig->prep.n; c++, k++)
+            if (lt) {
+                sr.p[k] = r->rig->lsrc[c];
+                sr.step[k] = r->rig->lstep[c];
+                any = true;
+            }
     }
     b->last = stream;
     b->used = true;
-    return run_sequence(b->seq, st, ot, stream, b->timer.begin_frame());
+    LutPass lp;
+    lp.jobs = b->d_lutjobs;
+    lp.njobs = b->n_lutjobs;
+    lp.blocks = b->lut_blocks;
+    lp.src = st;
+    return run_sequence(b->seq, sr, ot, stream, b->timer.begin_frame(), any ? &lp : nullptr);
 }
 
 int pano_batch_set_timing(pano_batch* b, int enable)

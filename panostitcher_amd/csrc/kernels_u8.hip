@@ -143,14 +143,10 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
 {
     __shared__ uint32_t in[kIH * kIW];
     __shared__ uint2 hs[kIH * kFTW];
-    __shared__ uint8_t lut_s[768];
     const Seg& S = seg_of(segs, nseg, blockIdx.x);
     const RenderDesc R = ldc(S.rd);
     const DownJob J = ldc((const DownJob*)S.jobs + (blockIdx.x - S.begin));
     const RemapCam cam = ldc(R.cams + J.src);
-    const bool use_lut = cam.lut >= 0 && (ldc((const uint32_t*)(R.luts + R.ncams * 768)) & 255u);
-    if (use_lut)
-        for (int e = threadIdx.x; e < 768; e += kDT) lut_s[e] = ldg(R.luts + cam.lut + e);
     const Plane P0 = ldc(R.planes + cam.plane);
     const Plane D = ldc(R.planes + J.dst);
     const uint8_t* sp = src.p[S.src_base + cam.src];
@@ -159,60 +155,132 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
     const int g0pitch = R.g0pitch;
     const int bx0 = 2 * J.X - 2, by0 = 2 * J.Y - 2;
     const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
-    // Software pipeline over kRounds rounds of kBatch pixels per thread: the
+    const float fsw = (float)cam.sw, fsh = (float)cam.sh;
+    // the staging region is one rectangle of the margined map storage
+    const char* mbase = (const char*)(cam.map + ((size_t)J.my * cam.mpitch + J.mx));
+    const uint32_t mrow = (uint32_t)cam.mpitch * 8u;
+    // Element e = threadIdx.x + k kDT of the region is (row, col) = (r, c);
+    // kDT = kStepR kIW + kStepC, so the next slot adds (kStepR, kStepC) with a carry.
+    constexpr int kStepR = kDT / kIW, kStepC = kDT % kIW;
+    // Software pipeline over kRounds rounds of kBatch points per thread: the
     // map points of round r+1 are in flight while round r gathers and combines.
     constexpr int kBatch = 3, kRounds = (kPer + kBatch - 1) / kBatch;
     float2 m[2][kBatch];
-    uint32_t fl[2][kBatch];
-    auto fetch_map = [&](int round, float2* mm, uint32_t* ff) {
+    int rr[kRounds * kBatch], cc[kRounds * kBatch];
+    {
+        int r = threadIdx.x / kIW, c = threadIdx.x - (threadIdx.x / kIW) * kIW;
+#pragma unroll
+        for (int k = 0; k < kRounds * kBatch; k++) {
+            rr[k] = r;
+            cc[k] = c;
+            r += kStepR;
+            c += kStepC;
+            if (c >= kIW) { c -= kIW; r++; }
+        }
+    }
+    auto fetch_map = [&](int round, float2* mm) {
 #pragma unroll
         for (int b = 0; b < kBatch; b++) {
-            int e = threadIdx.x + (round * kBatch + b) * kDT;
-            int r = e / kIW, c = e - r * kIW;
-            bool ok = e < kIH * kIW && r < ih && c < iw;
-            int bi = (J.my + r) * cam.mpitch + J.mx + c;
-            mm[b] = ok ? ldg_f2(cam.map + bi) : make_float2(-1.f, -1.f);
-            ff[b] = ok ? (uint32_t)ldg(cam.mask + bi) : 0u;
+            const int k = round * kBatch + b;
+            const bool ok = rr[k] < ih && cc[k] < iw;   // also false beyond kIH rows
+            const uint32_t off = __umul24((uint32_t)rr[k], mrow) + 8u * (uint32_t)cc[k];
+            mm[b] = ok ? ldg_f2((const float2*)(mbase + off)) : make_float2(__builtin_inff(), 0.f);
         }
     };
-    fetch_map(0, m[0], fl[0]);
-    __syncthreads();  // LUT staged
+    fetch_map(0, m[0]);
 #pragma unroll
     for (int round = 0; round < kRounds; round++) {
         const int cur = round & 1;
         uint32_t px[kBatch][4];
+        uint32_t o[kBatch][4];
 #pragma unroll
         for (int b = 0; b < kBatch; b++) {
-            double x = m[cur][b].x, y = m[cur][b].y;
-            if (x >= 0 && y >= 0 && x < cam.sw && y < cam.sh) {
-                uint32_t off[4];
-                bilinear_offsets(sstep, cam.sw, cam.sh, x, y, off);
+            const float x = __builtin_fabsf(m[cur][b].x), y = __builtin_fabsf(m[cur][b].y);
+            px[b][0] = px[b][1] = px[b][2] = px[b][3] = 0;
+            if (x < fsw && y < fsh) {
+                // samples of bilinearResampling: (x0, y0), its right neighbour in
+                // row-major order (the next row's first pixel at x0 = w - 1, the
+                // pixel itself at the image's last pixel), the same for row y1
+                const int x0 = (int)x, y0 = (int)y;
+                const bool ylast = y0 + 1 >= cam.sh;
+                uint32_t o0 = __umul24((uint32_t)y0, sstep) + 4u * (uint32_t)x0;
+                uint32_t o2 = ylast ? o0 : o0 + sstep;
+                uint32_t o1 = o0 + 4u, o3 = o2 + 4u;
+                if (x0 + 1 >= cam.sw) {
+                    const uint32_t nxt = sstep - 4u * (uint32_t)x0;
+                    o1 = ylast ? o0 : o0 + nxt;
+                    o3 = y0 + 2 >= cam.sh ? o2 : o2 + nxt;
+                }
+                o[b][0] = o0; o[b][1] = o1; o[b][2] = o2; o[b][3] = o3;
 #pragma unroll
-                for (int q = 0; q < 4; q++) px[b][q] = ldg((const uint32_t*)(sp + off[q]));
+                for (int q = 0; q < 4; q++) px[b][q] = ldg((const uint32_t*)(sp + o[b][q]));
             }
         }
-        if (round + 1 < kRounds) fetch_map(round + 1, m[cur ^ 1], fl[cur ^ 1]);
+        if (round + 1 < kRounds) fetch_map(round + 1, m[cur ^ 1]);
 #pragma unroll
         for (int b = 0; b < kBatch; b++) {
-            int e = threadIdx.x + (round * kBatch + b) * kDT;
+            const int k = round * kBatch + b;
+            const int e = threadIdx.x + k * kDT;
             if (e >= kIH * kIW) continue;
-            int r = e / kIW, c = e - r * kIW;
+            const int r = rr[k], c = cc[k];
+            const uint32_t bx = __float_as_uint(m[cur][b].x), by = __float_as_uint(m[cur][b].y);
+            const float x = __builtin_fabsf(m[cur][b].x), y = __builtin_fabsf(m[cur][b].y);
             uint32_t v = 0;
-            double x = m[cur][b].x, y = m[cur][b].y;
-            if (x >= 0 && y >= 0 && x < cam.sw && y < cam.sh)
-                v = bilinear_combine(x, y, px[b], use_lut ? lut_s : nullptr);
-            if (fl[cur][b] & kFlagValid) v |= 1u << 24;
+            if (x < fsw && y < fsh) {
+                bool tie = false;
+                v = bilinear_fast(__builtin_amdgcn_fractf(x), __builtin_amdgcn_fractf(y), px[b], tie);
+                if (tie) v = bilinear_combine((double)x, (double)y, px[b], nullptr);
+            }
+            v |= (by >> 31) << 24;   // kFlagValid
             // an owned core pixel (not a border-resolved copy of one) goes to the level-0 blend
-            if ((fl[cur][b] & kFlagOwned) && r >= 2 && c >= 2 && r < 2 + 2 * J.ny &&
-                c < 2 + 2 * J.nx) {
-                int vx = bx0 + c, vy = by0 + r;
-                if (vx < P0.cw && vy < P0.ch) stg(g0comp + (size_t)vy * g0pitch + vx, v);
+            if ((bx >> 31) && r >= 2 && c >= 2 && r < 2 + 2 * J.ny && c < 2 + 2 * J.nx) {
+                const int vx = bx0 + c, vy = by0 + r;
+                if (vx < P0.cw && vy < P0.ch) {
+                    // canvas byte offsets fit 32 bits (checked at prepare)
+                    const uint32_t go = 4u * (__umul24((uint32_t)vy, (uint32_t)g0pitch) + (uint32_t)vx);
+                    stg((uint32_t*)((char*)g0comp + go), v);
+                }
             }
             in[e] = v;
         }
     }
     __syncthreads();
     down_from_lds(in, hs, J, D);
+}
+
+// Gain LUTs applied to whole source frames (transform(), GainAdjust.cpp:
+// 295-346): byte c of a pixel becomes lut[256 c + byte], c = 0..2; byte 3 is kept.
+__global__ __launch_bounds__(256) void k_lut_src(const LutJob* __restrict__ jobs, int njobs,
+                                                 SrcTable src)
+{
+    __shared__ uint32_t lut_w[192];
+    int j = 0;
+    while (j + 1 < njobs && (int)blockIdx.x >= ldc(&jobs[j + 1].begin)) j++;
+    const LutJob J = ldc(jobs + j);
+    if (!ldg(J.flag)) return;   // LUTs off for this render's frame
+    if (threadIdx.x < 192) lut_w[threadIdx.x] = ldg((const uint32_t*)J.lut + threadIdx.x);
+    __syncthreads();
+    const uint8_t* L = (const uint8_t*)lut_w;
+    const uint8_t* in = src.p[J.src];
+    const uint32_t step = src.step[J.src];
+    const int y0 = ((int)blockIdx.x - J.begin) * J.rows;
+    const int y1 = min(y0 + J.rows, J.h);
+    for (int y = y0; y < y1; y++) {
+        const uint32_t* ir = (const uint32_t*)(in + (size_t)y * step);
+        uint32_t* orow = (uint32_t*)(J.out + (size_t)y * J.out_step);
+        for (int x = threadIdx.x; x < J.w; x += 4 * 256) {
+            uint32_t p[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) p[k] = x + 256 * k < J.w ? ldg(ir + x + 256 * k) : 0u;
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (x + 256 * k >= J.w) continue;
+                const uint32_t q = (uint32_t)L[p[k] & 255u] | ((uint32_t)L[256 + ((p[k] >> 8) & 255u)] << 8) |
+                                   ((uint32_t)L[512 + ((p[k] >> 16) & 255u)] << 16) | (p[k] & 0xff000000u);
+                stg(orow + x + 256 * k, q);
+            }
+        }
+    }
 }
 
 // ------------------------------------------------------- level j -> j+1 --
@@ -494,6 +562,13 @@ hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int leve
     default:
         return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_lut(const LutJob* jobs, int njobs, int blocks, const SrcTable& src, hipStream_t s)
+{
+    if (blocks <= 0 || njobs <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lut_src, dim3(blocks), dim3(256), 0, s, jobs, njobs, src);
     return hipGetLastError();
 }
 

@@ -100,10 +100,14 @@ constexpr int kMaxBatch = 32;   // renders per batched launch
 // border-resolved copies (WRAP columns, REFLECT_101 rows, invalid outside the
 // box) so that a tile's staging region is one contiguous rectangle; for the
 // 16-bit path they are box-local with the level-0 plane pitch.
+//
+// Fused storage packs the flags into the map: a point outside the source
+// image is stored as +inf, and the sign bit of x is kFlagOwned, the sign bit
+// of y kFlagValid (the map's own values are then |x|, |y|, never negative).
 constexpr int kMapMargin = 5;
 struct RemapCam {
     const float2* map;     // (x, y) map points
-    const uint8_t* mask;   // flags (kFlagValid | kFlagOwned)
+    const uint8_t* mask;   // flags (kFlagValid | kFlagOwned); null for fused storage
     int sw, sh;            // source image size
     int src;               // SrcTable index of its frame
     int lut;               // byte offset of its 3 x 256 LUT in the LUT buffer, -1 = none
@@ -116,6 +120,18 @@ struct RemapJob { int cam, lx, ly, nx, ny, pad0, pad1, pad2; };   // box-local t
 // position of the tile's staging region in the camera's margined map storage.
 struct DownJob { int src, dst, X, Y, nx, ny, mx, my; };
 struct TileJob { int X, Y, nx, ny, r, off, cnt, flags; };         // canvas tile at level j
+
+// Gain LUT pass of one source frame (transform(), GainAdjust.cpp:295-346):
+// blocks [begin, begin + ceil(h / rows)) map rows of SrcTable entry src into
+// out when the render's byte *flag is non-zero.  lut: 3 x 256 bytes (B, G, R),
+// 4-byte aligned.
+struct LutJob {
+    const uint8_t* lut;
+    const uint8_t* flag;
+    uint8_t* out;
+    uint32_t out_step;
+    int src, w, h, rows, begin;
+};
 
 enum TileFlags : int { kTileTop = 1 };
 

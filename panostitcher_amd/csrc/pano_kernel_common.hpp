@@ -212,6 +212,34 @@ __device__ __forceinline__ uint32_t bilinear_combine(double x, double y, const u
     return out;
 }
 
+// The same result from single-precision arithmetic where that is provably
+// equal.  With wx = x - x0 and wy = y - y0 (exact in float), the reference's
+// exact sum is V = H0 + (H1 - H0) wy with Hk = pk0 + (pk1 - pk0) wx, because
+// its weights are the exact products of (1 - wx, wx) and (1 - wy, wy).  Each
+// of the three fmas and the subtraction rounds once at magnitude < 256, so
+// |v - V| <= 3 * 2^-17; the FP64 sum of the reference is within 2^-40 of V.
+// Unless a half-integer lies within kTieEps of v, both round to the integer
+// nearest v.  tie is set when some channel is that close; the caller then
+// recomputes the pixel with bilinear_combine.
+constexpr float kTieEps = 0x1p-14f;
+__device__ __forceinline__ uint32_t bilinear_fast(float wx, float wy, const uint32_t p[4], bool& tie)
+{
+    uint32_t out = 0;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int sh8 = 8 * c;
+        float a = (float)((p[0] >> sh8) & 255u), b = (float)((p[1] >> sh8) & 255u);
+        float d = (float)((p[2] >> sh8) & 255u), e = (float)((p[3] >> sh8) & 255u);
+        float h0 = __fmaf_rn(b - a, wx, a);
+        float h1 = __fmaf_rn(e - d, wx, d);
+        float v = __fmaf_rn(h1 - h0, wy, h0);
+        float f = __builtin_amdgcn_fractf(v) - 0.5f;
+        tie |= __builtin_fabsf(f) < kTieEps;
+        out |= (uint32_t)(int)__builtin_rintf(v) << sh8;
+    }
+    return out;
+}
+
 // bilinearResampling<short,3> (ZReproject.cpp:101-130) with the LUT: the
 // right sample of each row is the next pixel in row-major order (the
 // reference's pointer walk on a continuous image, which at x0 = w - 1 is the

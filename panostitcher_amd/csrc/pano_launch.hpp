@@ -38,6 +38,10 @@ enum Stage : int { kStageRemapDown0 = 0, kStageDown = 1, kStageBlend = 2, kStage
 hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int level,
                       const SrcTable& src, const OutTable& outs, hipStream_t s);
 
+// Gain LUT pass over source frames ahead of k_remap_down0 (blocks = the
+// jobs' block total; src holds the frames the jobs' src fields index).
+hipError_t launch_lut(const LutJob* jobs, int njobs, int blocks, const SrcTable& src, hipStream_t s);
+
 // Zero-fill with a kernel (stays correct inside captured graphs).
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 

@@ -189,6 +189,34 @@ def test_render_inconsistent_maps_use_16bit_planes():
     assert_parity(from_dev(out)[..., :3], ref, exact=True, label="inconsistent")
 
 
+@pytest.mark.parametrize("q", [2, 4])
+def test_render_snapped_maps_hit_rounding_ties(q):
+    # Map points on a 1/q pixel grid make the exact bilinear sum a half-integer
+    # often (a cvRound tie); those pixels take the FP64 reference sequence.
+    wl = synth.workload("rect6", 0.2)
+    xs, ys, ms = synth.maps_and_masks(wl)
+    xs = [np.where(x >= 0, np.floor(x * q) / q, x).astype(np.float32) for x in xs]
+    ys = [np.where(y >= 0, np.floor(y * q) / q, y).astype(np.float32) for y in ys]
+    srcs = synth.frames(wl)
+    luts = synth.luts(wl)
+    ob = O.Blender()
+    assert ob.prepare(np.stack(ms), wl.levels)
+    ref = ob.render(srcs, xs, ys, luts, nthreads=8)
+    note(f"render snapped maps 1/{q} {wl.dst_w}x{wl.dst_h}")
+    r = P.PanoramaRender()
+    assert r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h),
+                          xs, ys, ms, wl.levels)
+    assert r.fused
+    out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    dsrc = [to_dev(synth.bgrx(s)) for s in srcs]
+    assert r.render(dsrc, out, luts)
+    assert_parity(from_dev(out)[..., :3], ref, exact=True, label=f"snapped 1/{q}")
+    # the same frame without LUTs
+    ref2 = ob.render(srcs, xs, ys, None, nthreads=8)
+    assert r.render(dsrc, out, None)
+    assert_parity(from_dev(out)[..., :3], ref2, exact=True, label=f"snapped 1/{q} no LUT")
+
+
 def test_batch_matches_alone_and_oracle():
     # Renders of different rigs and sizes stitched in one launch per stage give
     # the same canvases as each render alone, and the oracle's.
