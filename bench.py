@@ -119,6 +119,8 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the stitch path shards by stream, one rig per GPU, "
@@ -131,6 +133,16 @@ def main():
     from panostitcher_amd import synth
 
     P.lib()
+    if args.calib_copy_mib > 0:
+        # a copy kernel of known size: reads and writes calib_copy_mib MiB once
+        n = args.calib_copy_mib << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {args.calib_copy_mib} MiB")
+        del a, b
     wl = synth.workload(args.workload, args.scale)
     t0 = time.perf_counter()
     xs, ys, ms = synth.maps_and_masks(wl)
