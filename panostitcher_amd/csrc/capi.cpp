@@ -262,7 +262,7 @@ struct DeviceRig {
         if (fused) pl.add(P * 4);
         pl.add(sizeof(int) * prep.fin_owners.size());
         for (int j = 1; j <= L; j++) {
-            pl.add((size_t)prep.lw[j] * prep.lh[j] * 16);
+            pl.add((size_t)prep.lw[j] * prep.lh[j] * (fused ? 8 : 16));
             if (!prep.full) pl.add((size_t)prep.lw[j] * prep.lh[j] * 2);
         }
         pl.add(sizeof(Plane) * n * (L + 1));
@@ -317,7 +317,8 @@ struct DeviceRig {
             c.cw = prep.lw[j];
             c.ch = prep.lh[j];
             c.pitch = prep.lw[j];
-            c.c = (int4*)arena.take((size_t)c.cw * c.ch * 16);
+            if (fused) c.c16 = (uint2*)arena.take((size_t)c.cw * c.ch * 8);
+            else c.c = (int4*)arena.take((size_t)c.cw * c.ch * 16);
             if (!prep.full) {
                 c.wsum = (uint16_t*)arena.take((size_t)c.cw * c.ch * 2);
                 HIP_TRY(hipMemcpy(c.wsum, prep.wsum[j].data(), prep.wsum[j].size() * 2,
@@ -527,6 +528,7 @@ struct DeviceRig {
         const int n = prep.n, L = prep.levels;
         const size_t px = kind == kPixU8 ? 4 : 8;
         const size_t P = (size_t)prep.rows * prep.cols;
+        const size_t cb = prep.fused0 ? 8 : 16;   // bytes per C_j pixel
         size_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
         if (fused_maps)
             for (int i = 0; i < n; i++) s0 += (size_t)src_w[i] * src_h[i] * 4;
@@ -541,12 +543,14 @@ struct DeviceRig {
         for (int j = 1; j <= L; j++) {
             for (const TileJob& t : prep.blend[j]) {
                 size_t tp = (size_t)t.nx * t.ny;
-                s2 += tp * 16 + (prep.full ? 0 : tp * 2);                 // C_j, sum W
+                s2 += tp * cb + (prep.full ? 0 : tp * 2);                 // C_j, sum W
                 s2 += (size_t)t.cnt * tp * (px + 2);                       // G_j, W_j
-                if (j < L) s2 += (size_t)t.cnt * tp * px / 4 + tp * 16 / 4;  // G_j+1, C_j+1
+                if (j < L) s2 += (size_t)t.cnt * tp * px / 4 + tp * cb / 4;  // G_j+1, C_j+1
             }
         }
-        s3 = P * (1 + px + 4) + (L > 0 ? P * px / 4 + P * 16 / 4 : 0);
+        // level 0: owner label (a byte of the owners' samples on the 8-bit path),
+        // the owner's G_0, G_1 and C_1, the output
+        s3 = P * ((prep.fused0 ? 0 : 1) + px + 4) + (L > 0 ? P * px / 4 + P * cb / 4 : 0);
         model_bytes[0] = s0;
         model_bytes[1] = s1;
         model_bytes[2] = s2;
@@ -1000,7 +1004,9 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
     if (!r->ok) return PANO_ERR_STATE;
     DeviceRig& d = *r->rig;
     if (num != d.prep.n || !srcs || !src_steps || !dst) return PANO_ERR_ARG;
-    if (dst_step < (size_t)r->dst_w * 4 || dst_step % 4) return PANO_ERR_ARG;
+    if (dst_step < (size_t)r->dst_w * 4 || dst_step % 4 || dst_step >= (1u << 24) ||
+        dst_step * (uint64_t)r->dst_h > 0xffffffffu)
+        return PANO_ERR_ARG;
     SrcTable st{};
     for (int i = 0; i < num; i++) {
         if (!srcs[i] || src_steps[i] < (size_t)r->src_w[i] * 4 || src_steps[i] % 4 ||
@@ -1187,7 +1193,7 @@ int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* s
     for (size_t i = 0; i < b->renders.size(); i++) {
         pano_render* r = b->renders[i];
         if (!dsts[i] || dst_steps[i] < (size_t)r->dst_w * 4 || dst_steps[i] % 4 ||
-            dst_steps[i] > 0xffffffffu)
+            dst_steps[i] >= (1u << 24) || dst_steps[i] * (uint64_t)r->dst_h > 0xffffffffu)
             return PANO_ERR_ARG;
         ot.p[i] = dsts[i];
         ot.step[i] = (uint32_t)dst_steps[i];

@@ -76,6 +76,44 @@ __device__ __forceinline__ uint32_t load_u32(const Plane& P, int X, int Y)
     return i < 0 ? 0u : ldg((const uint32_t*)P.g + i);
 }
 
+// Pixel (X, Y) of a plane's box as a byte offset, or -1 outside it (box
+// pitches and rows stay below 2^24, the plane below 2^31 bytes).
+__device__ __forceinline__ int box_off32(const Plane& P, int X, int Y)
+{
+    const int ly = Y - P.y0;
+    int lx = X - P.x0;
+    if (lx < 0) lx += P.cw;
+    if ((unsigned)ly >= (unsigned)P.bh || (unsigned)lx >= (unsigned)P.bw) return -1;
+    return (int)(4u * (__umul24((uint32_t)ly, (uint32_t)P.pitch) + (uint32_t)lx));
+}
+
+__device__ __forceinline__ uint32_t load_u32_off(const Plane& P, int off)
+{
+    return off < 0 ? 0u : ldg((const uint32_t*)((const char*)P.g + off));
+}
+
+// Level-(j+1) staging region of a quad tile: element k of this thread is
+// canvas pixel (xs[k], ys[k]) (border-resolved), ok[k] whether it exists.
+constexpr int kSW = kQTW + 2, kSH = kQTH + 2;
+constexpr int kStageIt = (kSH * kSW + 255) / 256;
+struct StageCoords {
+    int xs[kStageIt], ys[kStageIt];
+    bool ok[kStageIt];
+};
+__device__ __forceinline__ StageCoords stage_coords(int sx0, int sy0, int cw, int ch)
+{
+    StageCoords S;
+#pragma unroll
+    for (int k = 0; k < kStageIt; k++) {
+        const int e = threadIdx.x + k * 256;
+        const int r = e / kSW, c = e - r * kSW;
+        S.ok[k] = e < kSH * kSW;
+        S.xs[k] = bwrap(sx0 + c, cw);
+        S.ys[k] = breflect(sy0 + r, ch);
+    }
+    return S;
+}
+
 // pyrUp of a 2x2 quad from packed 8-bit pixels staged in LDS: s points at the
 // quad's source pixel (x, y); stride is the LDS row length.  Outputs (even
 // row, even col), (even, odd), (odd, even), (odd, odd) as packed u8 lanes.
@@ -156,6 +194,7 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
     const int bx0 = 2 * J.X - 2, by0 = 2 * J.Y - 2;
     const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
     const float fsw = (float)cam.sw, fsh = (float)cam.sh;
+    const uint32_t owner_byte = (uint32_t)(J.src + 1) << 24;   // byte 3 of g0comp: owner + 1
     // the staging region is one rectangle of the margined map storage
     const char* mbase = (const char*)(cam.map + ((size_t)J.my * cam.mpitch + J.mx));
     const uint32_t mrow = (uint32_t)cam.mpitch * 8u;
@@ -238,7 +277,7 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
                 if (vx < P0.cw && vy < P0.ch) {
                     // canvas byte offsets fit 32 bits (checked at prepare)
                     const uint32_t go = 4u * (__umul24((uint32_t)vy, (uint32_t)g0pitch) + (uint32_t)vx);
-                    stg((uint32_t*)((char*)g0comp + go), v);
+                    stg((uint32_t*)((char*)g0comp + go), (v & 0xffffffu) | owner_byte);
                 }
             }
             in[e] = v;
@@ -316,9 +355,6 @@ __global__ __launch_bounds__(kDT) void k_down_u8(const Seg* __restrict__ segs, i
 
 // ------------------------------------------------------- blend, level >= 1 --
 
-constexpr int kSW = kQTW + 2, kSH = kQTH + 2;  // staged level-(j+1) region
-constexpr int kStageIt = (kSH * kSW + 255) / 256;
-
 __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, int nseg, int level)
 {
     __shared__ uint32_t gs[kMaxStage][kSH * kSW];
@@ -340,23 +376,20 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
     if (!R.full)
 #pragma unroll
         for (int d = 0; d < 4; d++)
-            if (in[d]) ws[d] = ldg(C.wsum + (Y + (d >> 1)) * C.pitch + X + (d & 1));
+            if (in[d]) ws[d] = ldg(C.wsum + (__umul24((uint32_t)(Y + (d >> 1)), (uint32_t)C.pitch) + X + (d & 1)));
     // stage up to kMaxStage cameras' level j+1 regions and C_{j+1} with every load in flight
+    const CanvasLevel C1 = ldc(R.canvas + (top ? level : level + 1));
+    const StageCoords SC = stage_coords(sx0, sy0, C1.cw, C1.ch);
     if (!top) {
-        const CanvasLevel C1 = ldc(R.canvas + level + 1);
         int4 cv[kStageIt];
 #pragma unroll
-        for (int k = 0; k < kStageIt; k++) {
-            int e = threadIdx.x + k * 256;
-            int r = e / kSW, c = e - r * kSW;
-            cv[k] = e < kSH * kSW
-                        ? ldg4(C1.c + breflect(sy0 + r, C1.ch) * C1.pitch + bwrap(sx0 + c, C1.cw))
-                        : make_int4(0, 0, 0, 0);
-        }
+        for (int k = 0; k < kStageIt; k++)
+            cv[k] = SC.ok[k] ? ldg_c16(C1.c16 + __umul24((uint32_t)SC.ys[k], (uint32_t)C1.pitch) + SC.xs[k])
+                             : make_int4(0, 0, 0, 0);
 #pragma unroll
         for (int k = 0; k < kStageIt; k++) {
             int e = threadIdx.x + k * 256;
-            if (e < kSH * kSW) cs[e] = cv[k];
+            if (SC.ok[k]) cs[e] = cv[k];
         }
     }
     int3 acc[4];
@@ -370,32 +403,27 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
             for (int s = 0; s < kMaxStage; s++) {
                 const Plane P1 = ldc(planes + ldc(camlist + J.off + g + (s < ng ? s : 0)) + 1);
 #pragma unroll
-                for (int k = 0; k < kStageIt; k++) {
-                    int e = threadIdx.x + k * 256;
-                    int r = e / kSW, c = e - r * kSW;
-                    gv[s][k] = (s < ng && e < kSH * kSW)
-                                   ? load_u32(P1, bwrap(sx0 + c, P1.cw), breflect(sy0 + r, P1.ch))
-                                   : 0u;
-                }
+                for (int k = 0; k < kStageIt; k++)
+                    gv[s][k] = (s < ng && SC.ok[k]) ? load_u32_off(P1, box_off32(P1, SC.xs[k], SC.ys[k])) : 0u;
             }
 #pragma unroll
             for (int s = 0; s < kMaxStage; s++)
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++) {
                     int e = threadIdx.x + k * 256;
-                    if (s < ng && e < kSH * kSW) gs[s][e] = gv[s][k];
+                    if (s < ng && SC.ok[k]) gs[s][e] = gv[s][k];
                 }
         }
         __syncthreads();
         for (int s = 0; s < ng; s++) {
             const Plane P = ldc(planes + ldc(camlist + J.off + g + s));
-            int idx[4], w[4];
+            int w[4];
             uint32_t gp[4];
 #pragma unroll
             for (int d = 0; d < 4; d++) {
-                idx[d] = in[d] ? box_index(P, X + (d & 1), Y + (d >> 1)) : -1;
-                w[d] = idx[d] >= 0 ? (int)ldg(P.w + idx[d]) : 0;
-                gp[d] = idx[d] >= 0 ? ldg((const uint32_t*)P.g + idx[d]) : 0u;
+                const int off = in[d] ? box_off32(P, X + (d & 1), Y + (d >> 1)) : -1;
+                w[d] = off >= 0 ? (int)ldg((const uint16_t*)((const char*)P.w + (off >> 1))) : 0;
+                gp[d] = load_u32_off(P, off);
             }
             uint32_t u[4] = {0, 0, 0, 0};
             if (!top) up_quad_u8(&gs[s][(qy + 1) * kSW + qx + 1], kSW, u);
@@ -430,7 +458,8 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
             a1 /= wv;
             a2 /= wv;
         }
-        stg4(C.c + Yp * C.pitch + Xp, make_int4(up[d].x + a0, up[d].y + a1, up[d].z + a2, 0));
+        stg_c16(C.c16 + (__umul24((uint32_t)Yp, (uint32_t)C.pitch) + Xp), up[d].x + a0, up[d].y + a1,
+                up[d].z + a2);
     }
 }
 
@@ -457,24 +486,24 @@ __global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, 
     for (int d = 0; d < 4; d++) {
         int Xp = X + (d & 1), Yp = Y + (d >> 1);
         in[d] = Xp < J.X + J.nx && Yp < J.Y + J.ny;
-        lab[d] = in[d] ? (int)ldg(R.label + Yp * R.label_pitch + Xp) : (int)kNoOwner;
-        g0[d] = in[d] ? ldg(R.g0comp + (size_t)Yp * R.g0pitch + Xp) : 0u;
+        // the owner's level-0 sample; byte 3 = owner + 1 (0: no owner, never written)
+        g0[d] = in[d] ? ldg((const uint32_t*)((const char*)R.g0comp +
+                                              4u * (__umul24((uint32_t)Yp, (uint32_t)R.g0pitch) + (uint32_t)Xp)))
+                      : 0u;
+        lab[d] = g0[d] >> 24 ? (int)(g0[d] >> 24) - 1 : (int)kNoOwner;
     }
+    const CanvasLevel C1 = ldc(R.canvas + (L > 0 ? 1 : 0));
+    const StageCoords SC = stage_coords(sx0, sy0, C1.cw, C1.ch);
     if (L > 0) {
-        const CanvasLevel C1 = ldc(R.canvas + 1);
         int4 cv[kStageIt];
 #pragma unroll
-        for (int k = 0; k < kStageIt; k++) {
-            int e = threadIdx.x + k * 256;
-            int r = e / kSW, c = e - r * kSW;
-            cv[k] = e < kSH * kSW
-                        ? ldg4(C1.c + breflect(sy0 + r, C1.ch) * C1.pitch + bwrap(sx0 + c, C1.cw))
-                        : make_int4(0, 0, 0, 0);
-        }
+        for (int k = 0; k < kStageIt; k++)
+            cv[k] = SC.ok[k] ? ldg_c16(C1.c16 + __umul24((uint32_t)SC.ys[k], (uint32_t)C1.pitch) + SC.xs[k])
+                             : make_int4(0, 0, 0, 0);
 #pragma unroll
         for (int k = 0; k < kStageIt; k++) {
             int e = threadIdx.x + k * 256;
-            if (e < kSH * kSW) cs[e] = cv[k];
+            if (SC.ok[k]) cs[e] = cv[k];
         }
     }
     int lap[4][3];
@@ -494,20 +523,15 @@ __global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, 
                 const int o = ldc(owners + J.off + g + (s < ng ? s : 0));
                 const Plane P1 = ldc(planes + o * (L + 1) + 1);
 #pragma unroll
-                for (int k = 0; k < kStageIt; k++) {
-                    int e = threadIdx.x + k * 256;
-                    int r = e / kSW, c = e - r * kSW;
-                    gv[s][k] = (s < ng && e < kSH * kSW)
-                                   ? load_u32(P1, bwrap(sx0 + c, P1.cw), breflect(sy0 + r, P1.ch))
-                                   : 0u;
-                }
+                for (int k = 0; k < kStageIt; k++)
+                    gv[s][k] = (s < ng && SC.ok[k]) ? load_u32_off(P1, box_off32(P1, SC.xs[k], SC.ys[k])) : 0u;
             }
 #pragma unroll
             for (int s = 0; s < kMaxStage; s++)
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++) {
                     int e = threadIdx.x + k * 256;
-                    if (s < ng && e < kSH * kSW) gs[s][e] = gv[s][k];
+                    if (s < ng && SC.ok[k]) gs[s][e] = gv[s][k];
                 }
             __syncthreads();
             for (int s = 0; s < ng; s++) {
@@ -536,7 +560,7 @@ __global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, 
         if (R.full || lab[d] != kNoOwner)
             o = (uint32_t)sat_u8(up[d].x + lap[d][0]) | ((uint32_t)sat_u8(up[d].y + lap[d][1]) << 8) |
                 ((uint32_t)sat_u8(up[d].z + lap[d][2]) << 16);
-        stg((uint32_t*)(out + (size_t)Yp * ostep) + Xp, o);
+        stg((uint32_t*)(out + (__umul24((uint32_t)Yp, ostep) + 4u * (uint32_t)Xp)), o);
     }
 }
 

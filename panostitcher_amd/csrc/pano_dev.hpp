@@ -53,7 +53,8 @@ struct Plane {
 };
 
 struct CanvasLevel {
-    int4* c;          // collapsed result C_j, row pitch in pixels
+    int4* c;          // collapsed result C_j, row pitch in pixels (16-bit planes)
+    uint2* c16;       // C_j as 4 x int16 (8-bit planes: |C_j| <= 256 (L - j + 1))
     uint16_t* wsum;   // sum of the cameras' weights, or nullptr when the masks cover the canvas
     int pitch;
     int cw, ch;
@@ -73,7 +74,7 @@ struct RenderDesc {
     const int* owners;           // level-0 owner cameras
     const RemapCam* cams;        // remap inputs
     const uint8_t* luts;         // n x 768 LUT bytes, then 1 byte: LUTs active this frame
-    uint32_t* g0comp;            // owners' level-0 samples
+    uint32_t* g0comp;            // owners' level-0 samples; byte 3 = owner + 1, 0 = no owner
     int label_pitch;
     int g0pitch;
     int full;                    // masks cover the whole canvas

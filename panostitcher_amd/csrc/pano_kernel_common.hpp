@@ -58,6 +58,18 @@ __device__ __forceinline__ short4 ldg_s4(const short4* p)
     pano_i16x4 v = *(const PANO_GLOBAL pano_i16x4*)p;
     return make_short4(v.x, v.y, v.z, v.w);
 }
+typedef unsigned int pano_u32x2 __attribute__((ext_vector_type(2)));
+// C_j pixel of the 8-bit path: channels 0..2 as int16.
+__device__ __forceinline__ int4 ldg_c16(const uint2* p)
+{
+    pano_u32x2 v = *(const PANO_GLOBAL pano_u32x2*)p;
+    return make_int4((int)(short)(v.x & 0xffffu), (int)v.x >> 16, (int)(short)(v.y & 0xffffu), 0);
+}
+__device__ __forceinline__ void stg_c16(uint2* p, int a, int b, int c)
+{
+    pano_u32x2 w = {((uint32_t)a & 0xffffu) | ((uint32_t)b << 16), (uint32_t)c & 0xffffu};
+    *(PANO_GLOBAL pano_u32x2*)p = w;
+}
 __device__ __forceinline__ float2 ldg_f2(const float2* p)
 {
     unsigned long long v = *(const PANO_GLOBAL unsigned long long*)p;
@@ -214,13 +226,14 @@ __device__ __forceinline__ uint32_t bilinear_combine(double x, double y, const u
 
 // The same result from single-precision arithmetic where that is provably
 // equal.  With wx = x - x0 and wy = y - y0 (exact in float), the reference's
-// exact sum is V = H0 + (H1 - H0) wy with Hk = pk0 + (pk1 - pk0) wx, because
+// exact sum is V = H0 + (H1 - H0) wy with Hk = pk0 (1 - wx) + pk1 wx, because
 // its weights are the exact products of (1 - wx, wx) and (1 - wy, wy).  Each
-// of the three fmas and the subtraction rounds once at magnitude < 256, so
-// |v - V| <= 3 * 2^-17; the FP64 sum of the reference is within 2^-40 of V.
-// Unless a half-integer lies within kTieEps of v, both round to the integer
-// nearest v.  tie is set when some channel is that close; the caller then
-// recomputes the pixel with bilinear_combine.
+// float operation rounds once at magnitude < 256 (half an ulp <= 2^-17):
+// hk = fma(wx, pk1, fma(-wx, pk0, pk0)) is within 2 * 2^-17 of Hk, and
+// v = fma(h1 - h0, wy, h0) within 4 * 2^-17 = 2^-15 of V; the FP64 sum of the
+// reference is within 2^-40 of V.  Unless a half-integer lies within kTieEps
+// of v, both round to the integer nearest v.  tie is set when some channel is
+// that close; the caller then recomputes the pixel with bilinear_combine.
 constexpr float kTieEps = 0x1p-14f;
 __device__ __forceinline__ uint32_t bilinear_fast(float wx, float wy, const uint32_t p[4], bool& tie)
 {
@@ -230,8 +243,8 @@ __device__ __forceinline__ uint32_t bilinear_fast(float wx, float wy, const uint
         const int sh8 = 8 * c;
         float a = (float)((p[0] >> sh8) & 255u), b = (float)((p[1] >> sh8) & 255u);
         float d = (float)((p[2] >> sh8) & 255u), e = (float)((p[3] >> sh8) & 255u);
-        float h0 = __fmaf_rn(b - a, wx, a);
-        float h1 = __fmaf_rn(e - d, wx, d);
+        float h0 = __fmaf_rn(wx, b, __fmaf_rn(-wx, a, a));
+        float h1 = __fmaf_rn(wx, e, __fmaf_rn(-wx, d, d));
         float v = __fmaf_rn(h1 - h0, wy, h0);
         float f = __builtin_amdgcn_fractf(v) - 0.5f;
         tie |= __builtin_fabsf(f) < kTieEps;
