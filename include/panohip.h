@@ -177,6 +177,28 @@ int64_t pano_render_model_bytes(const pano_render* r, int stage);
  * their masks and it has at least one pyramid level), 0 for the 16-bit path. */
 int pano_render_fused(const pano_render* r);
 
+/* ---- per-frame gain estimation ---------------------------------------------
+ * render() with gains estimated from the same frame: the overlap-means
+ * exposure estimate getTransformsMeanApproxSiftPanoPaper (Blend/Compensate.cpp:
+ * 501-563) with its anchor rescale (:565-575, as GainCompensate::prepare uses
+ * it, :614-646), optionally the tint estimate
+ * getTintTransformsMeanApproxMimicSiftPanoPaper (Blend/TintAdjust.cpp:123-215),
+ * both on the frame's 8-bit reprojection (before any LUT) and the rig's
+ * original masks, then the LUTs of getExposureColorOptimizeLUTs
+ * (Blend/ExposureColorOptimize.cpp:1425-1443) applied to the same frame.
+ * Statistics, solve and LUTs stay on the device (no host round trip).  Fused
+ * renders of up to 8 cameras (PANO_ERR_UNSUPPORTED otherwise). */
+#define PANO_GAIN_TINT 1
+int pano_render_render_autogain(pano_render* r, int num, const uint8_t* const* srcs,
+                                const size_t* src_steps, int flags, uint8_t* dst,
+                                size_t dst_step, hipStream_t stream);
+/* The last estimate (waits for the device): gains, red and blue ratio gains
+ * (num each, any may be NULL) and the anchor camera. */
+int pano_render_gains(const pano_render* r, double* gains, double* red, double* blue, int* anchor);
+/* The last estimate's sums (waits for the device): num x num x 5 int64 --
+ * count, gray, B, G, R of camera i's reprojection over mask_i & mask_j. */
+int pano_render_gain_sums(const pano_render* r, int64_t* sums);
+
 /* ---- batched render: several rigs per launch ------------------------------
  * One launch per pyramid stage covers every camera of every render in the
  * batch, each render's tiles taken from its own resident work lists (no

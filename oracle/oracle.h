@@ -24,6 +24,7 @@
  *   orc_get_lut / transform    rint(k*v) closed form incl. ties and clamps
  *   orc_true_num_levels        closed-form values
  *   orc_map_*                  geometric invariants of the equirect chain
+ *   orc_gain_* / orc_cv_solve  see gain.c
  */
 #ifndef PANO_ORACLE_H
 #define PANO_ORACLE_H
@@ -111,6 +112,20 @@ void orc_reproject_map_and_mask(const orc_photo_param* p, int src_w, int src_h, 
 int orc_render(orc_blender* b, int n, const uint8_t* const* srcs, const int* src_w,
                const int* src_h, const float* const* xmaps, const float* const* ymaps,
                const uint8_t* luts, uint8_t* dst, int nthreads);
+
+/* Per-frame gain estimation (gain.c).  bgr[i]: rows*cols*3 reprojected BGR
+ * bytes of camera i, masks[i]: its original mask (rows*cols bytes). */
+int orc_gray_bgr(int b, int g, int r);
+int orc_cv_solve(int n, double* A, const double* b, double* x);
+/* sums[(i*n + j)*5 + f], f = count, gray, B, G, R of image i over
+ * mask_i & mask_j (mask_i alone for j = i). */
+void orc_gain_sums(int n, int rows, int cols, const uint8_t* const* bgr,
+                   const uint8_t* const* masks, int64_t* sums);
+int orc_gain_from_sums(int n, const int64_t* sums, int tint, double* gains, double* rg,
+                       double* bg, int* max_index, uint8_t* luts);
+int orc_gain_estimate(int n, int rows, int cols, const uint8_t* const* bgr,
+                      const uint8_t* const* masks, int tint, double* gains, double* rg, double* bg,
+                      int* max_index, uint8_t* luts);
 
 #ifdef __cplusplus
 }

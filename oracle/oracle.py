@@ -69,6 +69,13 @@ def lib():
         L.orc_reproject_map_and_mask.argtypes = [vp, i, i, i, i, vp, vp, i]
         L.orc_render.argtypes = [vp, i, vp, vp, vp, vp, vp, vp, vp, i]
         L.orc_render.restype = i
+        L.orc_gray_bgr.argtypes = [i, i, i]
+        L.orc_gray_bgr.restype = i
+        L.orc_cv_solve.argtypes = [i, vp, vp, vp]
+        L.orc_cv_solve.restype = i
+        L.orc_gain_sums.argtypes = [i, i, i, vp, vp, vp]
+        L.orc_gain_from_sums.argtypes = [i, vp, i, vp, vp, vp, C.POINTER(i), vp]
+        L.orc_gain_from_sums.restype = i
         _lib = L
     return _lib
 
@@ -245,6 +252,52 @@ def transform(src: np.ndarray, lut3: np.ndarray) -> np.ndarray:
     out = np.zeros_like(src)
     lib().orc_transform(_p(src), _p(out), w, h, w * cn, cn, _p(lut3))
     return out
+
+
+def gray_bgr(b: int, g: int, r: int) -> int:
+    """cv::cvtColor(BGR2GRAY) of one 8-bit pixel."""
+    return lib().orc_gray_bgr(int(b), int(g), int(r))
+
+
+def cv_solve(A: np.ndarray, b: np.ndarray):
+    """cv::solve(A, b, x, DECOMP_LU) -> (ok, x)."""
+    n = A.shape[0]
+    A = np.ascontiguousarray(A, dtype=np.float64).copy()
+    b = np.ascontiguousarray(b, dtype=np.float64).reshape(n)
+    x = np.zeros(n, np.float64)
+    ok = lib().orc_cv_solve(n, _p(A), _p(b), _p(x))
+    return bool(ok), x
+
+
+def _ptrs(arrs):
+    return (C.c_void_p * len(arrs))(*[a.ctypes.data for a in arrs])
+
+
+def gain_sums(bgr, masks) -> np.ndarray:
+    """(n, n, 5) int64: count, gray, B, G, R of image i over mask_i & mask_j."""
+    bgr = [np.ascontiguousarray(a, dtype=np.uint8) for a in bgr]
+    masks = [np.ascontiguousarray(m, dtype=np.uint8) for m in masks]
+    n = len(bgr)
+    rows, cols = masks[0].shape
+    out = np.zeros((n, n, 5), np.int64)
+    lib().orc_gain_sums(n, rows, cols, _ptrs(bgr), _ptrs(masks), _p(out))
+    return out
+
+
+def gain_from_sums(sums: np.ndarray, tint: bool = True):
+    """-> (gains, rg, bg, max_index, luts (n, 3, 256))."""
+    sums = np.ascontiguousarray(sums, dtype=np.int64)
+    n = sums.shape[0]
+    g, rg, bg = (np.zeros(n, np.float64) for _ in range(3))
+    mi = C.c_int(0)
+    luts = np.zeros((n, 3, 256), np.uint8)
+    lib().orc_gain_from_sums(n, _p(sums), int(tint), _p(g), _p(rg), _p(bg), C.byref(mi), _p(luts))
+    return g, rg, bg, mi.value, luts
+
+
+def gain_estimate(bgr, masks, tint: bool = True):
+    """Gains from overlap means of reprojected BGR images (see gain.c)."""
+    return gain_from_sums(gain_sums(bgr, masks), tint)
 
 
 class PhotoParam(C.Structure):

@@ -91,6 +91,9 @@ def lib():
             "pano_render_set_timing": ([vp, i], i),
             "pano_render_stage_ms": ([vp, vp, C.POINTER(C.c_int)], i),
             "pano_render_fused": ([vp], i),
+            "pano_render_render_autogain": ([vp, i, pp, psz, i, vp, sz, vp], i),
+            "pano_render_gains": ([vp, vp, vp, vp, C.POINTER(C.c_int)], i),
+            "pano_render_gain_sums": ([vp, vp], i),
             "pano_batch_create": ([pp], i),
             "pano_batch_destroy": ([vp], None),
             "pano_batch_set": ([vp, i, pp], i),
@@ -106,6 +109,9 @@ def lib():
     return _lib
 
 
+PANO_GAIN_TINT = 1
+
+
 def exported_symbols():
     return [
         "pano_status_string", "pano_version", "pano_blender_create", "pano_blender_destroy",
@@ -117,7 +123,8 @@ def exported_symbols():
         "pano_render_prepare", "pano_render_prepare_maps", "pano_render_render",
         "pano_render_num_images", "pano_render_num_levels", "pano_render_mem_bytes",
         "pano_render_model_bytes", "pano_render_set_timing", "pano_render_stage_ms",
-        "pano_render_fused", "pano_batch_create", "pano_batch_destroy", "pano_batch_set",
+        "pano_render_fused", "pano_render_render_autogain", "pano_render_gains",
+        "pano_render_gain_sums", "pano_batch_create", "pano_batch_destroy", "pano_batch_set",
         "pano_batch_render", "pano_batch_set_timing", "pano_batch_stage_ms",
     ]
 
@@ -417,6 +424,36 @@ class PanoramaRender:
             return False
         _check(st, "render")
         return True
+
+    def render_autogain(self, srcs, out, tint=True, stream=None) -> bool:
+        """render() with gains estimated from this frame's overlap means
+        (GainCompensate / getTransformsMeanApproxSiftPanoPaper, optionally the
+        tint ratios) and applied as LUTs, all on the device."""
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in srcs])
+        op, ostep = _dev_ptr_step(out)
+        st = lib().pano_render_render_autogain(self._h, len(srcs), _ptrs(ptrs), _sizes(steps),
+                                               PANO_GAIN_TINT if tint else 0, C.c_void_p(op),
+                                               ostep, _stream(stream))
+        if st == -2:
+            return False
+        _check(st, "render_autogain")
+        return True
+
+    def gains(self):
+        """The last estimate: (gains, red ratio gains, blue ratio gains, anchor)."""
+        n = lib().pano_render_num_images(self._h)
+        g, r, b = (np.zeros(n, np.float64) for _ in range(3))
+        a = C.c_int()
+        _check(lib().pano_render_gains(self._h, g.ctypes.data, r.ctypes.data, b.ctypes.data,
+                                       C.byref(a)), "gains")
+        return g, r, b, a.value
+
+    def gain_sums(self) -> np.ndarray:
+        """The last estimate's (n, n, 5) int64 sums: count, gray, B, G, R."""
+        n = lib().pano_render_num_images(self._h)
+        out = np.zeros((n, n, 5), np.int64)
+        _check(lib().pano_render_gain_sums(self._h, out.ctypes.data), "gain sums")
+        return out
 
 
 class Batch:

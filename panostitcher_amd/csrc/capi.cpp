@@ -187,9 +187,10 @@ struct LutPass {
 // first launch and after the remap (with the LUT pass), down, blend and final
 // stages.  st: the frames the remap reads (LUT-mapped copies where lut is given).
 int run_sequence(const std::vector<Launch>& seq, const SrcTable& st, const OutTable& ot,
-                 hipStream_t s, std::vector<hipEvent_t>* ev, const LutPass* lut = nullptr)
+                 hipStream_t s, std::vector<hipEvent_t>* ev, const LutPass* lut = nullptr,
+                 bool ev0_recorded = false)
 {
-    if (ev) HIP_TRY(hipEventRecord((*ev)[0], s));
+    if (ev && !ev0_recorded) HIP_TRY(hipEventRecord((*ev)[0], s));
     if (lut) HIP_TRY(launch_lut(lut->jobs, lut->njobs, lut->blocks, lut->src, s));
     int prev = kStageRemapDown0;
     for (const Launch& l : seq) {
@@ -236,6 +237,9 @@ struct DeviceRig {
     std::vector<LutJob> lutjobs;            // fused: LUT pass over this render's frames
     LutJob* d_lutjobs = nullptr;
     int lut_blocks = 0;
+    uint8_t* d_cover = nullptr;             // gain estimation: masks covering each canvas pixel
+    unsigned long long* d_gstats = nullptr;
+    double* d_gains = nullptr;
     size_t model_bytes[8] = {};
 
     int levels() const { return prep.levels; }
@@ -260,6 +264,12 @@ struct DeviceRig {
                 if (j > 0) pl.add(g.pixels() * 2);
             }
         if (fused) pl.add(P * 4);
+        const bool gain = fused && xmaps && n <= kGainMaxCams;
+        if (gain) {
+            pl.add(P);
+            pl.add(sizeof(unsigned long long) * n * n * 5);
+            pl.add(sizeof(double) * (3 * n + 1));
+        }
         pl.add(sizeof(int) * prep.fin_owners.size());
         for (int j = 1; j <= L; j++) {
             pl.add((size_t)prep.lw[j] * prep.lh[j] * (fused ? 8 : 16));
@@ -464,8 +474,28 @@ struct DeviceRig {
             }
             if (err) return err;
         }
+        if (gain) {
+            std::vector<uint8_t> cover(P, 0);
+            for (int i = 0; i < n; i++)
+                for (int y = 0; y < prep.rows; y++) {
+                    const uint8_t* m = masks[i] + (size_t)y * mstep[i];
+                    uint8_t* c = cover.data() + (size_t)y * prep.cols;
+                    for (int x = 0; x < prep.cols; x++)
+                        if (m[x]) c[x] |= (uint8_t)(1u << i);
+                }
+            d_cover = upload_vec(arena, cover, err);
+            d_gstats = (unsigned long long*)arena.take(sizeof(unsigned long long) * n * n * 5);
+            d_gains = (double*)arena.take(sizeof(double) * (3 * n + 1));
+            if (!d_gstats || !d_gains) return PANO_ERR_NOMEM;
+            HIP_TRY(hipMemset(d_gstats, 0, sizeof(unsigned long long) * n * n * 5));
+            HIP_TRY(hipMemset(d_gains, 0, sizeof(double) * (3 * n + 1)));
+            if (err) return err;
+        }
         RenderDesc rd{};
         rd.label = d_label;
+        rd.cover = d_cover;
+        rd.gstats = d_gstats;
+        rd.gains = d_gains;
         rd.planes = d_planes;
         rd.canvas = d_canvas;
         rd.camlist = d_camlist;
@@ -996,18 +1026,22 @@ int pano_render_prepare(pano_render* r, int num, const pano_photo_param* params,
                                     kp.data(), max_levels);
 }
 
-int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
-                       const size_t* src_steps, const uint8_t* luts, uint8_t* dst,
-                       size_t dst_step, hipStream_t stream)
+}  // extern "C"
+
+namespace {
+
+// Argument checks of a render call; fills the frame and output tables.
+int render_args(const pano_render* r, int num, const uint8_t* const* srcs, const size_t* src_steps,
+                uint8_t* dst, size_t dst_step, SrcTable& st, OutTable& ot)
 {
     if (!r) return PANO_ERR_ARG;
     if (!r->ok) return PANO_ERR_STATE;
-    DeviceRig& d = *r->rig;
+    const DeviceRig& d = *r->rig;
     if (num != d.prep.n || !srcs || !src_steps || !dst) return PANO_ERR_ARG;
     if (dst_step < (size_t)r->dst_w * 4 || dst_step % 4 || dst_step >= (1u << 24) ||
         dst_step * (uint64_t)r->dst_h > 0xffffffffu)
         return PANO_ERR_ARG;
-    SrcTable st{};
+    st = SrcTable{};
     for (int i = 0; i < num; i++) {
         if (!srcs[i] || src_steps[i] < (size_t)r->src_w[i] * 4 || src_steps[i] % 4 ||
             src_steps[i] * (uint64_t)r->src_h[i] > 0xffffffffu || src_steps[i] >= (1u << 24))
@@ -1015,25 +1049,48 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
         st.p[i] = srcs[i];
         st.step[i] = (uint32_t)src_steps[i];
     }
-    std::vector<hipEvent_t>* ev = r->timer.begin_frame();
-    OutTable ot{};
+    ot = OutTable{};
     ot.p[0] = dst;
     ot.step[0] = (uint32_t)dst_step;
+    return PANO_OK;
+}
+
+// The fused sequence with the LUT pass: the remap reads the LUT-mapped copies.
+int run_fused_with_luts(DeviceRig& d, const SrcTable& st, const OutTable& ot, hipStream_t stream,
+                        std::vector<hipEvent_t>* ev, bool ev0_recorded)
+{
+    LutPass lp;
+    lp.jobs = d.d_lutjobs;
+    lp.njobs = (int)d.lutjobs.size();
+    lp.blocks = d.lut_blocks;
+    lp.src = st;
+    SrcTable sr = st;
+    for (int i = 0; i < d.prep.n; i++) {
+        sr.p[i] = d.lsrc[i];
+        sr.step[i] = d.lstep[i];
+    }
+    return run_sequence(d.seq, sr, ot, stream, ev, &lp, ev0_recorded);
+}
+
+}  // namespace
+
+extern "C" {
+
+int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
+                       const size_t* src_steps, const uint8_t* luts, uint8_t* dst,
+                       size_t dst_step, hipStream_t stream)
+{
+    SrcTable st;
+    OutTable ot;
+    int err = render_args(r, num, srcs, src_steps, dst, dst_step, st, ot);
+    if (err) return err;
+    DeviceRig& d = *r->rig;
+    std::vector<hipEvent_t>* ev = r->timer.begin_frame();
     if (d.prep.fused0) {
         int e = r->upload_luts(luts, stream);
         if (e) return e;
         if (!luts) return run_sequence(d.seq, st, ot, stream, ev);
-        LutPass lp;
-        lp.jobs = d.d_lutjobs;
-        lp.njobs = (int)d.lutjobs.size();
-        lp.blocks = d.lut_blocks;
-        lp.src = st;
-        SrcTable sr = st;
-        for (int i = 0; i < num; i++) {
-            sr.p[i] = d.lsrc[i];
-            sr.step[i] = d.lstep[i];
-        }
-        return run_sequence(d.seq, sr, ot, stream, ev, &lp);
+        return run_fused_with_luts(d, st, ot, stream, ev, false);
     }
     if (luts) {
         int e = r->ring.upload(luts, (size_t)num * 768, d.d_luts, stream);
@@ -1044,6 +1101,56 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
                          d.d_luts, luts != nullptr, st, stream));
     if (ev) HIP_TRY(hipEventRecord((*ev)[1], stream));
     return d.run_pyramid(-1, st, ot, stream, ev);
+}
+
+int pano_render_render_autogain(pano_render* r, int num, const uint8_t* const* srcs,
+                                const size_t* src_steps, int flags, uint8_t* dst,
+                                size_t dst_step, hipStream_t stream)
+{
+    SrcTable st;
+    OutTable ot;
+    int err = render_args(r, num, srcs, src_steps, dst, dst_step, st, ot);
+    if (err) return err;
+    DeviceRig& d = *r->rig;
+    if (!d.prep.fused0 || !d.d_cover || d.seq.empty()) return PANO_ERR_UNSUPPORTED;
+    if (flags & ~PANO_GAIN_TINT) return PANO_ERR_ARG;
+    std::vector<hipEvent_t>* ev = r->timer.begin_frame();
+    if (ev) HIP_TRY(hipEventRecord((*ev)[0], stream));
+    const int n = d.prep.n;
+    HIP_TRY(launch_zero(d.d_gstats, sizeof(unsigned long long) * n * n * 5, stream));
+    HIP_TRY(launch_gain(d.seq[0].segs, 1, d.seq[0].blocks, d.d_rd,
+                        (flags & PANO_GAIN_TINT) ? kGainTint : 0, st, stream));
+    r->lut_state = 1;   // the solve set the device's "apply LUTs" byte
+    return run_fused_with_luts(d, st, ot, stream, ev, true);
+}
+
+int pano_render_gains(const pano_render* r, double* gains, double* red, double* blue, int* anchor)
+{
+    if (!r) return PANO_ERR_ARG;
+    if (!r->ok) return PANO_ERR_STATE;
+    const DeviceRig& d = *r->rig;
+    if (!d.d_gains) return PANO_ERR_UNSUPPORTED;
+    const int n = d.prep.n;
+    std::vector<double> h(3 * n + 1);
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(h.data(), d.d_gains, sizeof(double) * h.size(), hipMemcpyDeviceToHost));
+    if (gains) memcpy(gains, h.data(), sizeof(double) * n);
+    if (red) memcpy(red, h.data() + n, sizeof(double) * n);
+    if (blue) memcpy(blue, h.data() + 2 * n, sizeof(double) * n);
+    if (anchor) *anchor = (int)h[3 * n];
+    return PANO_OK;
+}
+
+int pano_render_gain_sums(const pano_render* r, int64_t* sums)
+{
+    if (!r || !sums) return PANO_ERR_ARG;
+    if (!r->ok) return PANO_ERR_STATE;
+    const DeviceRig& d = *r->rig;
+    if (!d.d_gstats) return PANO_ERR_UNSUPPORTED;
+    const int n = d.prep.n;
+    HIP_TRY(hipDeviceSynchronize());
+    HIP_TRY(hipMemcpy(sums, d.d_gstats, sizeof(int64_t) * n * n * 5, hipMemcpyDeviceToHost));
+    return PANO_OK;
 }
 
 int pano_render_set_timing(pano_render* r, int enable)

@@ -166,14 +166,6 @@ __device__ __forceinline__ int lane(uint32_t v, int c) { return (int)((v >> (8 *
 
 }  // namespace
 
-// Segment of a batched launch that owns block b (segments are few and ordered).
-__device__ __forceinline__ const Seg& seg_of(const Seg* segs, int nseg, int b)
-{
-    int s = 0;
-    while (s + 1 < nseg && b >= segs[s + 1].begin) s++;
-    return segs[s];
-}
-
 // ------------------------------------------------------------ level 0 -> 1 --
 
 __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ segs, int nseg,
@@ -237,20 +229,7 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
             const float x = __builtin_fabsf(m[cur][b].x), y = __builtin_fabsf(m[cur][b].y);
             px[b][0] = px[b][1] = px[b][2] = px[b][3] = 0;
             if (x < fsw && y < fsh) {
-                // samples of bilinearResampling: (x0, y0), its right neighbour in
-                // row-major order (the next row's first pixel at x0 = w - 1, the
-                // pixel itself at the image's last pixel), the same for row y1
-                const int x0 = (int)x, y0 = (int)y;
-                const bool ylast = y0 + 1 >= cam.sh;
-                uint32_t o0 = __umul24((uint32_t)y0, sstep) + 4u * (uint32_t)x0;
-                uint32_t o2 = ylast ? o0 : o0 + sstep;
-                uint32_t o1 = o0 + 4u, o3 = o2 + 4u;
-                if (x0 + 1 >= cam.sw) {
-                    const uint32_t nxt = sstep - 4u * (uint32_t)x0;
-                    o1 = ylast ? o0 : o0 + nxt;
-                    o3 = y0 + 2 >= cam.sh ? o2 : o2 + nxt;
-                }
-                o[b][0] = o0; o[b][1] = o1; o[b][2] = o2; o[b][3] = o3;
+                tap_offsets(sstep, cam.sw, cam.sh, x, y, o[b]);
 #pragma unroll
                 for (int q = 0; q < 4; q++) px[b][q] = ldg((const uint32_t*)(sp + o[b][q]));
             }

@@ -224,6 +224,38 @@ __device__ __forceinline__ uint32_t bilinear_combine(double x, double y, const u
     return out;
 }
 
+// Byte offsets of the four samples of bilinearResampling at a point inside
+// the image, for the 32-bit frame addressing of the fused path (row step and
+// frame size below 2^24 and 2^32 bytes): (x0, y0), its right neighbour in
+// row-major order (the next row's first pixel at x0 = w - 1, the pixel itself
+// at the image's last pixel), then the same for row y1 = min(y0 + 1, h - 1).
+__device__ __forceinline__ void tap_offsets(uint32_t sstep, int sw, int sh, float x, float y,
+                                            uint32_t o[4])
+{
+    const int x0 = (int)x, y0 = (int)y;
+    const bool ylast = y0 + 1 >= sh;
+    const uint32_t o0 = __umul24((uint32_t)y0, sstep) + 4u * (uint32_t)x0;
+    const uint32_t o2 = ylast ? o0 : o0 + sstep;
+    uint32_t o1 = o0 + 4u, o3 = o2 + 4u;
+    if (x0 + 1 >= sw) {
+        const uint32_t nxt = sstep - 4u * (uint32_t)x0;
+        o1 = ylast ? o0 : o0 + nxt;
+        o3 = y0 + 2 >= sh ? o2 : o2 + nxt;
+    }
+    o[0] = o0;
+    o[1] = o1;
+    o[2] = o2;
+    o[3] = o3;
+}
+
+// Segment of a batched launch that owns block b (segments are few and ordered).
+__device__ __forceinline__ const Seg& seg_of(const Seg* segs, int nseg, int b)
+{
+    int s = 0;
+    while (s + 1 < nseg && b >= segs[s + 1].begin) s++;
+    return segs[s];
+}
+
 // The same result from single-precision arithmetic where that is provably
 // equal.  With wx = x - x0 and wy = y - y0 (exact in float), the reference's
 // exact sum is V = H0 + (H1 - H0) wy with Hk = pk0 (1 - wx) + pk1 wx, because

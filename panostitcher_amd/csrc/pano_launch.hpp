@@ -42,6 +42,13 @@ hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int leve
 // jobs' block total; src holds the frames the jobs' src fields index).
 hipError_t launch_lut(const LutJob* jobs, int njobs, int blocks, const SrcTable& src, hipStream_t s);
 
+// Per-frame gain estimation (kernels_gain.hip): statistics over the fused
+// level-0 jobs of segs (the remap launch's table), then the solve writing the
+// render's gains and LUTs (kGainTint: also the tint ratios).  The statistics
+// buffer must be zero.
+hipError_t launch_gain(const Seg* segs, int nseg, int blocks, const RenderDesc* rd, int flags,
+                       const SrcTable& src, hipStream_t s);
+
 // Zero-fill with a kernel (stays correct inside captured graphs).
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
 

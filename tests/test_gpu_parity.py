@@ -217,6 +217,46 @@ def test_render_snapped_maps_hit_rounding_ties(q):
     assert_parity(from_dev(out)[..., :3], ref2, exact=True, label=f"snapped 1/{q} no LUT")
 
 
+@pytest.mark.parametrize("name,scale,tint", [("fisheye2", 0.25, True), ("rect6", 0.2, False),
+                                             ("rect6", 0.2, True), ("mixed8", 0.125, True)])
+def test_render_autogain_matches_oracle(name, scale, tint):
+    # Per-frame gains from overlap means, estimated and applied on the device,
+    # against the oracle's estimate on its own 8-bit reprojections.
+    wl = synth.workload(name, scale)
+    xs, ys, ms = synth.maps_and_masks(wl)
+    r = P.PanoramaRender()
+    assert r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h),
+                          xs, ys, ms, wl.levels)
+    assert r.fused
+    ob = O.Blender()
+    assert ob.prepare(np.stack(ms), wl.levels)
+    out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    rng = np.random.default_rng(17)
+    for frame in range(2):
+        k = rng.uniform(0.8, 1.2, size=(wl.n, 3)) if tint else rng.uniform(0.8, 1.2, size=(wl.n, 1))
+        srcs = [np.clip(np.rint(s * k[i]), 0, 255).astype(np.uint8)
+                for i, s in enumerate(synth.frames(wl, frame))]
+        rep = [O.reproject_to16s(s, x, y, nthreads=8).astype(np.uint8)
+               for s, x, y in zip(srcs, xs, ys)]
+        sums = O.gain_sums(rep, ms)
+        g, rg, bg, anchor, luts = O.gain_from_sums(sums, tint)
+        ref = ob.render(srcs, xs, ys, luts, nthreads=8)
+        note(f"autogain {name}@{scale} tint={tint} frame={frame} n={wl.n}")
+        assert r.render_autogain([to_dev(synth.bgrx(s)) for s in srcs], out, tint=tint)
+        np.testing.assert_array_equal(r.gain_sums(), sums)
+        dg, drg, dbg, danchor = r.gains()
+        assert danchor == anchor
+        np.testing.assert_array_equal(dg, g)
+        np.testing.assert_array_equal(drg, rg)
+        np.testing.assert_array_equal(dbg, bg)
+        assert_parity(from_dev(out)[..., :3], ref, exact=True, label=f"autogain {name} f{frame}")
+    # a plain render afterwards applies no LUTs again
+    srcs = synth.frames(wl, 0)
+    assert r.render([to_dev(synth.bgrx(s)) for s in srcs], out, None)
+    assert_parity(from_dev(out)[..., :3], ob.render(srcs, xs, ys, None, nthreads=8), exact=True,
+                  label="plain after autogain")
+
+
 def test_batch_matches_alone_and_oracle():
     # Renders of different rigs and sizes stitched in one launch per stage give
     # the same canvases as each render alone, and the oracle's.
