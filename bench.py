@@ -1,20 +1,34 @@
 #!/usr/bin/env python3
 """Throughput of the MI355X stitch path on one GPU.
 
-Workload of the headline metric: 8 cameras of three sizes (2 x 2560x1440
-rectilinear 90 deg, 4 x 1920x1080 rectilinear 100 deg, 2 x 1280x1280 fisheye
-190 deg) remapped and multiband-blended into a 7680x3840 equirect canvas with
-9 pyramid levels.  A step stitches one frame of each of S streams (default
-16) with one launch per pyramid stage (pano_batch_render); inputs (camera
-frames, maps, seams, weights) are resident in HBM before timing starts and the
-timed region is K back-to-back steps on one stream, bracketed by device
-synchronisation.  value = output Mpix/s over all streams.  The same line
-reports one stream alone (render()) and the plain drop-in (reproject +
-blender, the reference's GPU API usage).
+Default (--config c3), the headline metric's workload: 8 cameras of three sizes
+(2 x 2560x1440 rectilinear 90 deg, 4 x 1920x1080 rectilinear 100 deg, 2 x
+1280x1280 fisheye 190 deg) remapped and multiband-blended into a 7680x3840
+equirect canvas with 9 pyramid levels.  A step stitches one frame of each of S
+streams (default 16) with one launch per pyramid stage (pano_batch_render);
+inputs (camera frames, maps, seams, weights) are resident in HBM before timing
+starts and the timed region is K back-to-back steps on one stream, bracketed by
+device synchronisation.  value = output Mpix/s over all streams.  The same line
+reports one stream alone (render()) and the plain drop-in (reproject + blender,
+the reference's GPU API usage).
 
-Prints one JSON line (see the field list in main()).  The CPU baseline is the
-oracle's restatement of CPUPanoramaRender::render with row-parallel reprojection
-and per-camera pyramid threads, timed on a bounded sample of the same frames.
+Other configurations of the same path (one JSON line each, same fields):
+  c1  2 fisheyes 1920x1920 195 deg -> 2048x1024, L=5, one frame per step
+  c2  6 rectilinear 1920x1080 100 deg -> 3840x1920, L=7, gains as LUTs: the
+      plain drop-in (transform + reproject + blender) and the fused render
+  c4  the serving loop (pano_server): 16 concurrent streams of seeded rigs (2-8
+      cameras, 1280x720 to 3840x2160, fisheye or rectilinear, outputs 2048x1024 /
+      3840x1920 / 7680x3840 with 5 / 7 / 9 levels) admitted by an HBM budget; one
+      stream leaves and another joins at Poisson times (mean 0.1 s); each stream
+      submits 30 frames/s; then the same 16 streams kept saturated
+  c5  4 fisheyes 3840x2160 200 deg -> 7680x3840, L=10, gains re-estimated from
+      overlap means every frame under a seeded brightness random walk
+
+The roofline object gives the dominant stage's algorithmic bytes (the 16-bit
+stage model below) over its mean device time, measured with HIP events on the
+stream the kernels run on.  The CPU baseline is the oracle's restatement of
+CPUPanoramaRender::render with row-parallel reprojection and per-camera pyramid
+threads, timed on a bounded sample of the same frames.
 """
 from __future__ import annotations
 
@@ -31,10 +45,43 @@ sys.path.insert(0, ROOT)
 
 METRIC = "stitched output Mpix/s (8-cam→7680x3840) + p99 frame latency, 16 streams"
 HBM_PEAK_GBS = 8000.0
+STAGES = ("remap", "down", "blend", "final")
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stage_model_bytes(wl, xs, ys, ms, L):
+    """Algorithmic bytes of one frame per stage, the 16-bit stage model: every
+    plane written once and read once per consuming stage, 3-channel int16 planes
+    on each camera's support, int16 weights.  With S_i source pixels, P_i valid
+    map points of camera i, q_j = 4^-j and P canvas pixels:
+      remap  3 S_i + P_i (8 map + 6 G0 write + 6 G0 read) + q_1 P_i (6 G1 write + 4 alpha)
+      down   P_i sum_{1<=j<L} q_j 6 + P_i sum_{2<=j<=L} q_j (6 + 4)
+      blend  P_i sum_{1<=j<=L} q_j (6 G + 2 W) + P_i sum_{1<=j<L} q_{j+1} 6
+             + P sum_{1<=j<=L} q_j (6 R write + 6 R read + 6 C write) + P sum_{1<=j<L} q_{j+1} 6
+      final  P_i (6 + 2) + q_1 P_i 6 + P (12 + 3) + q_1 P 6
+    plus, when the masks do not cover the canvas, 4 q_j P per level for the
+    weight sums and 1 P for the outside mask.  The sum is
+    3 S + P_i K_cam + P K_out with K_cam ~ 38 B and K_out ~ 23 B."""
+    S = sum(c.width * c.height for c in wl.cameras)
+    Pi = float(sum(int(((x >= 0) & (y >= 0) & (x < c.width) & (y < c.height)).sum())
+                   for x, y, c in zip(xs, ys, wl.cameras)))
+    P = float(wl.dst_w * wl.dst_h)
+    full = bool(np.logical_or.reduce([m > 0 for m in ms]).all())
+    q = [4.0 ** -j for j in range(L + 1)]
+    st = {
+        "remap": 3 * S + Pi * 20 + q[1] * Pi * 10,
+        "down": Pi * sum(q[j] * 6 for j in range(1, L)) + Pi * sum(q[j] * 10 for j in range(2, L + 1)),
+        "blend": Pi * sum(q[j] * 8 for j in range(1, L + 1)) + Pi * sum(q[j + 1] * 6 for j in range(1, L))
+        + P * sum(q[j] * 18 for j in range(1, L + 1)) + P * sum(q[j + 1] * 6 for j in range(1, L)),
+        "final": Pi * 8 + q[1] * Pi * 6 + P * 15 + q[1] * P * 6,
+    }
+    if not full:
+        st["blend"] += P * sum(q[j] * 4 for j in range(1, L + 1))
+        st["final"] += P * 5
+    return st
 
 
 def cpu_baseline(wl, srcs, xs, ys, ms, luts, frames: int):
@@ -82,10 +129,10 @@ def timed(fn, steps, warmup):
     return time.perf_counter() - t0
 
 
-def plain_dropin(wl, xs, ys, ms, dsrc, steps):
-    """The reference's GPU API used as CudaPanoramaRender2 does: cudaTransform +
-    cudaReprojectTo16S per camera into full-canvas 16SC4 images, then
-    CudaTilingMultibandBlendFast::blend (RicohUtil.cpp:1301-1313)."""
+def plain_dropin(wl, xs, ys, ms, dsrc, steps, luts=None):
+    """The reference's GPU API used as CudaPanoramaRender2 does: cudaTransform
+    (when gains are given) + cudaReprojectTo16S per camera into full-canvas 16SC4
+    images, then CudaTilingMultibandBlendFast::blend (RicohUtil.cpp:1301-1313)."""
     import torch
 
     import panostitcher_amd as P
@@ -95,87 +142,125 @@ def plain_dropin(wl, xs, ys, ms, dsrc, steps):
     dx = [torch.from_numpy(x).cuda() for x in xs]
     dy = [torch.from_numpy(y).cuda() for y in ys]
     rep = [torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.int16, device="cuda") for _ in xs]
+    adj = [torch.empty_like(s) for s in dsrc] if luts is not None else dsrc
     out = torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
 
     def frame():
         for i in range(wl.n):
-            P.cuda_reproject_to16s(dsrc[i], dx[i], dy[i], rep[i])
+            if luts is not None:
+                P.cuda_transform(dsrc[i], luts[i], adj[i])
+            P.cuda_reproject_to16s(adj[i], dx[i], dy[i], rep[i])
         b.blend(rep, out)
 
     dt = timed(frame, steps, 2)
     return {"value": round(wl.out_pixels * steps / dt / 1e6, 2), "unit": "Mpix/s",
             "ms_per_frame": round(dt * 1e3 / steps, 4),
-            "path": "reproject_to16s per camera + blender.blend, full-canvas 16-bit planes"}
+            "path": ("transform + " if luts is not None else "") +
+                    "reproject_to16s per camera + blender.blend, full-canvas 16-bit planes"}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="mixed8")
-    ap.add_argument("--scale", type=float, default=1.0)
-    ap.add_argument("--streams", type=int, default=16)
-    ap.add_argument("--cpu-frames", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-plain", action="store_true")
-    ap.add_argument("--calib-copy-mib", type=int, default=0,
-                    help="first run a plain device copy of this many MiB (counter calibration)")
-    args = ap.parse_args()
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the stitch path shards by stream, one rig per GPU, "
-              "and this build runs on a single MI355X; run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
+def roofline(model, per_stage_ms, frames_per_launch, own_bytes, traffic):
+    """The dominant stage's launch: algorithmic bytes (16-bit stage model) over
+    its device time; also the whole frame, and the same stage under the 8-bit
+    planes this build stores (pano_render_model_bytes)."""
+    dom = max(per_stage_ms, key=per_stage_ms.get)
+    launch_s = per_stage_ms[dom] * frames_per_launch * 1e-3
+    algo = model[dom] * frames_per_launch
+    achieved = algo / launch_s / 1e9 if launch_s else 0.0
+    total_s = sum(per_stage_ms.values()) * 1e-3
+    frame_gbs = sum(model.values()) / total_s / 1e9 if total_s else 0.0
+    own = own_bytes[STAGES.index(dom)] * frames_per_launch
+    return {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": traffic.get(dom) if isinstance(traffic, dict) else None,
+        "algorithmic_bytes": int(algo),
+        "model": "16-bit stage model (3-channel int16 planes, int16 weights)",
+        "stage_ms_per_frame": {k: round(v, 4) for k, v in per_stage_ms.items()},
+        "frame_model_bytes": int(sum(model.values())),
+        "frame_achieved": round(frame_gbs, 1),
+        "frame_frac": round(frame_gbs / HBM_PEAK_GBS, 4),
+        "achieved_8bit_model": round(own / launch_s / 1e9, 1) if launch_s else 0.0,
+    }
 
-    import torch
 
+def base_line(metric, value, steps, warmup, ms_per_step, config):
+    return {
+        "metric": metric,
+        "value": round(value, 2),
+        "unit": "Mpix/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": config,
+    }
+
+
+def setup(name, scale):
     import panostitcher_amd as P
     from panostitcher_amd import synth
 
     P.lib()
-    if args.calib_copy_mib > 0:
-        # a copy kernel of known size: reads and writes calib_copy_mib MiB once
-        n = args.calib_copy_mib << 20
-        a = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        log(f"calibration copy of {args.calib_copy_mib} MiB")
-        del a, b
-    wl = synth.workload(args.workload, args.scale)
+    wl = synth.workload(name, scale)
     t0 = time.perf_counter()
     xs, ys, ms = synth.maps_and_masks(wl)
-    frames = synth.frames(wl)
-    srcs = [synth.bgrx(f) for f in frames]
+    srcs = [synth.bgrx(f) for f in synth.frames(wl)]
     luts = synth.luts(wl)
     log(f"inputs {wl.name} {wl.dst_w}x{wl.dst_h} n={wl.n} in {time.perf_counter() - t0:.1f}s")
+    return wl, xs, ys, ms, srcs, luts
+
+
+def new_render(wl, xs, ys, ms):
+    import panostitcher_amd as P
+
+    r = P.PanoramaRender()
+    ok = r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h), xs, ys,
+                        ms, wl.levels)
+    if not ok or not r.fused:
+        raise SystemExit("prepare failed")
+    return r
+
+
+def per_frame_stages(obj):
+    stages, nf = obj.stage_ms()
+    return {k: v / max(nf, 1) for k, v in stages.items()}
+
+
+def run_batched(args):
+    """c3: S streams of the mixed8 rig, one launch per stage per step."""
+    import torch
+
+    import panostitcher_amd as P
+
+    wl, xs, ys, ms, srcs, luts = setup(args.workload or "mixed8", args.scale)
     S = max(1, args.streams)
     renders = []
     t0 = time.perf_counter()
-    for k in range(S):
-        r = P.PanoramaRender()
-        ok = r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h), xs,
-                            ys, ms, wl.levels)
-        if not ok or not r.fused:
-            raise SystemExit("prepare failed")
-        renders.append(r)
-    log(f"prepared {S} renders in {time.perf_counter() - t0:.1f}s, levels {renders[0].num_levels}, "
-        f"{renders[0].mem_bytes() / 1e9:.2f} GB each")
+    for _ in range(S):
+        renders.append(new_render(wl, xs, ys, ms))
+    r0 = renders[0]
+    log(f"prepared {S} renders in {time.perf_counter() - t0:.1f}s, levels {r0.num_levels}, "
+        f"{r0.mem_bytes() / 1e9:.2f} GB each")
     dsrc = [torch.from_numpy(s).cuda() for s in srcs]
-    # every stream has its own resident frame buffers and canvas
     stream_src = [dsrc] + [[t.clone() for t in dsrc] for _ in range(S - 1)]
     outs = [torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda") for _ in range(S)]
     stream = torch.cuda.current_stream()
 
-    # single stream: render() of one rig per call
-    r0 = renders[0]
     dt1 = timed(lambda: r0.render(dsrc, outs[0], luts, stream), args.steps, args.warmup)
     single = {"value": round(wl.out_pixels * args.steps / dt1 / 1e6, 2), "unit": "Mpix/s",
               "ms_per_frame": round(dt1 * 1e3 / args.steps, 4)}
     log(f"single stream {single}")
 
-    # batched: one launch per stage for the frames of all S streams
     batch = P.Batch(renders)
     lut_list = [luts] * S
 
@@ -193,77 +278,359 @@ def main():
         step()
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    stages, nframes = batch.stage_ms()
-    # per-batch device latency: every stream's frame is ready when its batch ends
+    per_step = per_frame_stages(batch)
     lat = []
     for _ in range(min(args.steps, 20)):
         step()
         st, _ = batch.stage_ms()
         lat.append(sum(st.values()))
     batch.set_timing(False)
-    log(f"batched {S} streams: {args.steps} steps in {elapsed * 1e3:.1f} ms; stages {stages}")
+    log(f"batched {S} streams: {args.steps} steps in {elapsed * 1e3:.1f} ms; per step {per_step}")
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = S * wl.out_pixels * args.steps / elapsed / 1e6
-    per_stage = {k: v / max(nframes, 1) for k, v in stages.items()}
-    dom = max(per_stage, key=per_stage.get)
-    stage_idx = P.PanoramaRender.STAGES.index(dom)
-    dom_bytes = S * r0.model_bytes(stage_idx)
-    achieved = dom_bytes / (per_stage[dom] * 1e-3) / 1e9
-    traffic = read_traffic(wl.name)
-    roofline = {
-        "bound": "hbm",
-        "kernel": dom,
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": traffic.get(dom) if isinstance(traffic, dict) else None,
-        "algorithmic_bytes": dom_bytes,
-        "stage_ms": {k: round(v, 4) for k, v in per_stage.items()},
-        "step_model_bytes": S * r0.model_bytes(4),
-        "step_model_gbs": round(S * r0.model_bytes(4) / (ms_per_step * 1e-3) / 1e9, 1),
-    }
+    per_frame = {k: v / S for k, v in per_step.items()}
+    model = stage_model_bytes(wl, xs, ys, ms, r0.num_levels)
+    own = [r0.model_bytes(k) for k in range(4)]
+    roof = roofline(model, per_frame, S, own, read_traffic(wl.name))
     plain = None
     if not args.no_plain:
         log("plain drop-in ...")
-        plain = plain_dropin(wl, xs, ys, ms, dsrc, max(3, args.steps // 4))
+        plain = plain_dropin(wl, xs, ys, ms, dsrc, max(3, args.steps // 4), luts)
         log(f"plain {plain}")
     cpu = None
     if not args.no_cpu and args.cpu_frames > 0:
         log("cpu baseline ...")
         cpu = cpu_baseline(wl, srcs, xs, ys, ms, luts, args.cpu_frames)
-    line = {
-        "metric": METRIC,
-        "value": round(value, 2),
-        "unit": "Mpix/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": round(ms_per_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic",
-        "config": {
-            "workload": f"{wl.name}: 8 cams (2x2560x1440 rect 90, 4x1920x1080 rect 100, "
-                        f"2x1280x1280 fisheye 190) -> {wl.dst_w}x{wl.dst_h} equirect, "
-                        f"L={r0.num_levels}; a step stitches one frame of each of {S} streams "
-                        f"with one launch per pyramid stage",
-            "streams": S,
-            "frames_per_step": S,
-            "parallelism": "single GPU",
-        },
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 4),
-                       "p99": round(float(np.percentile(lat, 99)), 4),
-                       "streams": S,
-                       "note": "device time of a step; each stream's frame completes with it"},
-        "single_stream": single,
-        "plain_dropin": plain,
-        "roofline": roofline,
-        "cpu_baseline": cpu,
-    }
+    line = base_line(METRIC, value, args.steps, args.warmup, ms_per_step, {
+        "workload": f"{wl.name}: 8 cams (2x2560x1440 rect 90, 4x1920x1080 rect 100, "
+                    f"2x1280x1280 fisheye 190) -> {wl.dst_w}x{wl.dst_h} equirect, "
+                    f"L={r0.num_levels}; a step stitches one frame of each of {S} streams "
+                    f"with one launch per pyramid stage",
+        "streams": S,
+        "frames_per_step": S,
+        "parallelism": "single GPU",
+    })
+    line["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 4),
+                          "p99": round(float(np.percentile(lat, 99)), 4),
+                          "streams": S,
+                          "note": "device time of a step; each stream's frame completes with it"}
+    line["single_stream"] = single
+    line["plain_dropin"] = plain
+    if plain:
+        line["vs_plain_dropin"] = round(value / plain["value"], 3)
+    line["roofline"] = roof
+    line["cpu_baseline"] = cpu
+    return line
+
+
+def timed_stages(r, fn, steps):
+    """Per-frame device latencies (one synchronising frame at a time), then the
+    mean per-stage device time over back-to-back frames."""
+    r.set_timing(True)
+    r.stage_ms()
+    lat = []
+    for _ in range(min(steps, 50)):
+        fn()
+        st, _ = r.stage_ms()
+        lat.append(sum(st.values()))
+    for _ in range(steps):
+        fn()
+    per_frame = per_frame_stages(r)
+    r.set_timing(False)
+    return lat, per_frame
+
+
+def run_single(args, name, label, with_plain):
+    """c1 / c2: one rig, one frame per step through render()."""
+    import torch
+
+    wl, xs, ys, ms, srcs, luts = setup(name, args.scale)
+    r = new_render(wl, xs, ys, ms)
+    dsrc = [torch.from_numpy(s).cuda() for s in srcs]
+    out = torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+
+    def fn():
+        r.render(dsrc, out, luts, stream)
+
+    dt = timed(fn, args.steps, args.warmup)
+    lat, per_frame = timed_stages(r, fn, args.steps)
+    value = wl.out_pixels * args.steps / dt / 1e6
+    model = stage_model_bytes(wl, xs, ys, ms, r.num_levels)
+    own = [r.model_bytes(k) for k in range(4)]
+    line = base_line(f"stitched output Mpix/s ({label})", value, args.steps, args.warmup,
+                     dt * 1e3 / args.steps, {"workload": f"{wl.name} -> {wl.dst_w}x{wl.dst_h}, "
+                                             f"L={r.num_levels}, {wl.n} cams, one frame per step",
+                                             "streams": 1, "parallelism": "single GPU"})
+    line["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 4),
+                          "p99": round(float(np.percentile(lat, 99)), 4), "streams": 1}
+    line["roofline"] = roofline(model, per_frame, 1, own, read_traffic(wl.name))
+    if with_plain:
+        line["plain_dropin"] = plain_dropin(wl, xs, ys, ms, dsrc, max(3, args.steps // 2), luts)
+    line["cpu_baseline"] = None
+    if not args.no_cpu and args.cpu_frames > 0:
+        line["cpu_baseline"] = cpu_baseline(wl, srcs, xs, ys, ms, luts, args.cpu_frames)
+    return line
+
+
+def run_autogain(args):
+    """c5: fisheye4, gains re-estimated from overlap means every frame."""
+    import torch
+
+    wl, xs, ys, ms, srcs, _ = setup("fisheye4", args.scale)
+    r = new_render(wl, xs, ys, ms)
+    rng = np.random.default_rng(20261203 + 5000)
+    F = 8
+    k = np.ones(wl.n)
+    base = [torch.from_numpy(s).cuda() for s in srcs]
+    frames = []
+    for _ in range(F):
+        # seeded random walk in brightness, N(0, 0.01) per camera per frame
+        k = np.clip(k + rng.normal(0, 0.01, size=wl.n), 0.5, 1.5)
+        fr = []
+        for i, t in enumerate(base):
+            v = (t.float() * float(k[i])).round().clamp(0, 255).to(torch.uint8)
+            v[..., 3] = 0
+            fr.append(v)
+        frames.append(fr)
+    out = torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    it = [0]
+
+    def fn():
+        r.render_autogain(frames[it[0] % F], out, tint=True, stream=stream)
+        it[0] += 1
+
+    dt = timed(fn, args.steps, args.warmup)
+    lat, per_frame = timed_stages(r, fn, args.steps)
+    gains, red, blue, anchor = r.gains()
+    value = wl.out_pixels * args.steps / dt / 1e6
+    model = stage_model_bytes(wl, xs, ys, ms, r.num_levels)
+    own = [r.model_bytes(k) for k in range(4)]
+    line = base_line("stitched output Mpix/s (4 fisheye 3840x2160 -> 7680x3840, per-frame gains)",
+                     value, args.steps, args.warmup, dt * 1e3 / args.steps,
+                     {"workload": f"fisheye4 -> {wl.dst_w}x{wl.dst_h}, L={r.num_levels}, gains and "
+                                  f"tint re-estimated every frame (overlap means, on the device), "
+                                  f"brightness random walk N(0, 0.01) over {F} resident frames",
+                      "streams": 1, "parallelism": "single GPU"})
+    line["latency_ms"] = {"p50": round(float(np.percentile(lat, 50)), 4),
+                          "p99": round(float(np.percentile(lat, 99)), 4), "streams": 1,
+                          "budget_60fps": 16.667}
+    line["roofline"] = roofline(model, per_frame, 1, own, None)
+    line["roofline"]["note"] = "remap stage time includes the gain statistics, solve and LUT pass"
+    line["last_gains"] = {"exposure": [round(float(g), 5) for g in gains], "anchor": anchor}
+    line["cpu_baseline"] = None
+    return line
+
+
+def run_served(args):
+    """c4: the serving loop under a paced 30 fps load with stream churn, then
+    saturated."""
+    import concurrent.futures as cf
+
+    import torch
+
+    import panostitcher_amd as P
+    from panostitcher_amd import synth
+
+    R, S, fps = 24, 16, 30.0
+    budget = int(args.hbm_budget_gb * 2 ** 30)
+    t0 = time.perf_counter()
+
+    def prep(k):
+        wl = synth.served_rig(k)
+        xs, ys, ms = synth.maps_and_masks(wl)
+        r = P.PanoramaRender()
+        ok = r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h), xs,
+                            ys, ms, wl.levels)
+        if not ok or not r.fused:
+            raise SystemExit(f"prepare of rig {k} failed")
+        return wl, r
+
+    with cf.ThreadPoolExecutor(4) as ex:
+        rigs = list(ex.map(prep, range(R)))
+    log(f"prepared {R} rigs in {time.perf_counter() - t0:.1f}s, "
+        f"{sum(r.mem_bytes() for _, r in rigs) / 1e9:.1f} GB resident")
+    gen = torch.Generator(device="cuda").manual_seed(20261203 + 4000)
+    srcs, outs = [], []
+    for wl, _ in rigs:
+        fr = []
+        for c in wl.cameras:
+            t = torch.randint(0, 256, (c.height, c.width, 4), dtype=torch.uint8, device="cuda",
+                              generator=gen)
+            t[..., 3] = 0
+            fr.append(t)
+        srcs.append(fr)
+        outs.append([torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+                     for _ in range(3)])
+    torch.cuda.synchronize()
+    srv = P.Server(budget, max_inflight=2)
+    rng = np.random.default_rng(20261203 + 4001)
+    active, pending_join = {}, []
+    rejected = 0
+    for k in range(R):
+        if len(active) == S:
+            break
+        sid = srv.admit(rigs[k][1])
+        if sid is None:
+            rejected += 1
+            continue
+        active[k] = sid
+    sent = {}
+    frame_no = {k: 0 for k in range(R)}
+
+    def submit(k, now):
+        sid = active[k]
+        i = frame_no[k]
+        frame_no[k] += 1
+        sent[(sid, i)] = (k, now)
+        srv.submit(sid, srcs[k], outs[k][i % 3], tag=i)
+
+    def collect(done, lat, pix):
+        for sid, tag, ms, st, bf in done:
+            if st != 0:
+                raise SystemExit(f"frame failed: status {st}")
+            k, _ = sent.pop((sid, tag))
+            lat.append(ms)
+            pix.append(rigs[k][0].out_pixels)
+            batch_sizes.append(bf)
+
+    # warm-up: one frame of every rig alone
+    for k in list(active):
+        submit(k, time.perf_counter())
+    srv.drain()
+    batch_sizes = []
+    srv.poll(1024)
+    sent.clear()
+
+    # paced phase: 30 fps per stream, one leave + one join at Poisson times
+    lat, pix = [], []
+    T = args.serve_seconds
+    start = time.perf_counter()
+    due = {k: start + rng.uniform(0, 1 / fps) for k in active}
+    next_event = start + rng.exponential(0.1)
+    joins = leaves = 0
+    while True:
+        now = time.perf_counter()
+        if now - start >= T:
+            break
+        if now >= next_event:
+            k_out = int(rng.choice(list(active)))
+            srv.retire(active.pop(k_out))
+            leaves += 1
+            idle = [k for k in range(R) if k not in active and k != k_out]
+            rng.shuffle(idle)
+            for k_in in idle:
+                sid = srv.admit(rigs[k_in][1])
+                if sid is None:
+                    rejected += 1
+                    continue
+                active[k_in] = sid
+                due[k_in] = time.perf_counter()
+                joins += 1
+                break
+            next_event += rng.exponential(0.1)
+        for k in list(active):
+            while due[k] <= now:
+                submit(k, now)
+                due[k] += 1 / fps
+        collect(srv.poll(1024, wait_ms=1), lat, pix)
+    srv.drain()
+    collect(srv.poll(4096), lat, pix)
+    paced_s = time.perf_counter() - start
+    paced = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+             "p99_ms": round(float(np.percentile(lat, 99)), 3),
+             "frames": len(lat), "seconds": round(paced_s, 2),
+             "mpix_s": round(sum(pix) / paced_s / 1e6, 1),
+             "joins": joins, "leaves": leaves, "rejected_admissions": rejected,
+             "mean_batch_frames": round(float(np.mean(batch_sizes)), 2) if batch_sizes else 0}
+    log(f"paced {paced}")
+
+    # saturated phase: the current streams keep two frames queued each
+    lat, pix, batch_sizes = [], [], []
+    for k in list(active):
+        submit(k, 0)
+        submit(k, 0)
+    start = time.perf_counter()
+    while time.perf_counter() - start < args.serve_seconds / 2:
+        done = srv.poll(1024, wait_ms=2)
+        for sid, tag, ms, st, bf in done:
+            k = sent[(sid, tag)][0]
+            if k in active:
+                submit(k, 0)
+        collect(done, lat, pix)
+    srv.drain()
+    collect(srv.poll(4096), lat, pix)
+    sat_s = time.perf_counter() - start
+    stats = srv.stats()
+    srv.close()
+    value = sum(pix) / sat_s / 1e6
+    line = base_line("served aggregate output Mpix/s (16 streams, mixed rigs) + p99 frame latency",
+                     value, len(lat), 0, sat_s * 1e3 / max(len(lat), 1),
+                     {"workload": f"served mix: {R} seeded rigs (2-8 cams, 1280x720..3840x2160, "
+                                  f"fisheye/rect) -> 2048x1024 / 3840x1920 / 7680x3840, L 5/7/9; "
+                                  f"{S} concurrent streams, HBM budget {args.hbm_budget_gb} GiB",
+                      "streams": len(active), "parallelism": "single GPU"})
+    line["latency_ms"] = {"p50": paced["p50_ms"], "p99": paced["p99_ms"], "streams": S,
+                          "load": "30 frames/s per stream with churn"}
+    line["paced"] = paced
+    line["saturated"] = {"p50_ms": round(float(np.percentile(lat, 50)), 3),
+                         "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                         "frames": len(lat), "seconds": round(sat_s, 2),
+                         "mean_batch_frames": round(float(np.mean(batch_sizes)), 2)}
+    line["server"] = stats
+    line["cpu_baseline"] = None
+    return line
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--serve-seconds", type=float, default=6.0)
+    ap.add_argument("--hbm-budget-gb", type=float, default=10.0)
+    ap.add_argument("--workload", default=None)
+    ap.add_argument("--scale", type=float, default=1.0)
+    ap.add_argument("--streams", type=int, default=16)
+    ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run a plain device copy of this many MiB (counter calibration)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: the stitch path shards by stream, one rig per GPU, "
+              "and this build runs on a single MI355X; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import panostitcher_amd as P
+
+    P.lib()
+    if args.calib_copy_mib > 0:
+        # a copy kernel of known size: reads and writes calib_copy_mib MiB once
+        n = args.calib_copy_mib << 20
+        a = torch.empty(n, dtype=torch.uint8, device="cuda").fill_(1)
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {args.calib_copy_mib} MiB")
+        del a, b
+    if args.config == "c3":
+        line = run_batched(args)
+    elif args.config == "c1":
+        line = run_single(args, "fisheye2", "2 fisheye 1920x1920 -> 2048x1024", False)
+    elif args.config == "c2":
+        line = run_single(args, "rect6", "6 rect 1920x1080 -> 3840x1920, gains", True)
+    elif args.config == "c4":
+        line = run_served(args)
+    else:
+        line = run_autogain(args)
     print(json.dumps(line), flush=True)
 
 

@@ -205,8 +205,10 @@ int pano_render_gain_sums(const pano_render* r, int64_t* sums);
  * counterpart in the reference, which stitches one rig per call,
  * RicohUtil.cpp:1207-1339; this is for servers that stitch many streams).
  * Up to 32 renders and 128 cameras in total, all on the fused 8-bit path.
- * The renders must outlive the batch.  pano_batch_set waits for the batch's
- * earlier work before replacing its tables. */
+ * The renders must outlive the batch.  pano_batch_set only builds the new
+ * launch tables on the host; the next pano_batch_render uploads them
+ * (stream-ordered) into one of 4 device table slots, waiting only when all 4
+ * still serve renders in flight. */
 typedef struct pano_batch pano_batch;
 
 int pano_batch_create(pano_batch** out);
@@ -220,6 +222,46 @@ int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* s
                       hipStream_t stream);
 int pano_batch_set_timing(pano_batch* b, int enable);
 int pano_batch_stage_ms(pano_batch* b, double* ms4, int* frames);
+
+/* ---- serving loop ----------------------------------------------------------
+ * Replaces the per-stream worker threads of the live task
+ * (Task/PanoramaLiveStreamTask2.cpp:340-350, 979-983, each calling
+ * CudaPanoramaRender2::render for its own stream) with one server: prepared
+ * renders are admitted as streams while their resident device memory
+ * (pano_render_mem_bytes) fits the HBM budget; submitted frames are batched
+ * continuously across streams -- whenever fewer than max_inflight batches are
+ * queued on the server's stream, the oldest pending frame of every stream with
+ * nothing in flight is launched in one pano_batch_render -- and each frame's
+ * submit-to-ready latency is reported.  Inputs and outputs of a submitted frame
+ * must stay valid until it is reported done.  Calls are thread-safe. */
+typedef struct pano_server pano_server;
+typedef struct pano_server_done {
+    uint64_t tag;         /* as submitted */
+    double latency_ms;    /* submit -> output ready (host clock) */
+    int stream_id;
+    int status;           /* PANO_OK or the launch error */
+    int batch_frames;     /* frames in the batch it ran in */
+    int pad;
+} pano_server_done;
+
+int pano_server_create(pano_server** out, int64_t hbm_budget_bytes, int max_inflight);
+/* Waits for the batches in flight; frames not yet launched are dropped. */
+void pano_server_destroy(pano_server* s);
+/* PANO_ERR_BUDGET when the render does not fit the remaining budget. */
+int pano_server_admit(pano_server* s, pano_render* r, int* stream_id);
+/* Waits for the stream's submitted frames, then releases its budget. */
+int pano_server_retire(pano_server* s, int stream_id);
+/* srcs/src_steps: the stream's cameras (device BGRx); luts: host num x 3 x 256
+ * (copied) or NULL; dst: device BGRx canvas. */
+int pano_server_submit(pano_server* s, int stream_id, const uint8_t* const* srcs,
+                       const size_t* src_steps, const uint8_t* luts, uint8_t* dst, size_t dst_step,
+                       uint64_t tag);
+/* Moves up to max_out completed frames into out (waiting up to wait_ms for
+ * the first); returns their number or a negative status. */
+int pano_server_poll(pano_server* s, pano_server_done* out, int max_out, int wait_ms);
+/* Waits until every submitted frame is done. */
+int pano_server_drain(pano_server* s);
+int pano_server_stats(pano_server* s, int64_t* used_bytes, int64_t* frames_done, int64_t* batches);
 
 /* Per-stage device timing of render() (no counterpart in the reference, whose
  * timers are host-side, Tool/Timer.h:8-143): when enabled, HIP events are

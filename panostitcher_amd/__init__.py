@@ -29,6 +29,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libpanohip.so")
 
 PANO_OK = 0
+PANO_ERR_BUDGET = -6
 PANO_8UC4 = 24
 PANO_16SC4 = 27
 
@@ -94,6 +95,14 @@ def lib():
             "pano_render_render_autogain": ([vp, i, pp, psz, i, vp, sz, vp], i),
             "pano_render_gains": ([vp, vp, vp, vp, C.POINTER(C.c_int)], i),
             "pano_render_gain_sums": ([vp, vp], i),
+            "pano_server_create": ([pp, i64, i], i),
+            "pano_server_destroy": ([vp], None),
+            "pano_server_admit": ([vp, vp, C.POINTER(C.c_int)], i),
+            "pano_server_retire": ([vp, i], i),
+            "pano_server_submit": ([vp, i, pp, psz, vp, vp, sz, C.c_uint64], i),
+            "pano_server_poll": ([vp, vp, i, i], i),
+            "pano_server_drain": ([vp], i),
+            "pano_server_stats": ([vp, vp, vp, vp], i),
             "pano_batch_create": ([pp], i),
             "pano_batch_destroy": ([vp], None),
             "pano_batch_set": ([vp, i, pp], i),
@@ -126,6 +135,8 @@ def exported_symbols():
         "pano_render_fused", "pano_render_render_autogain", "pano_render_gains",
         "pano_render_gain_sums", "pano_batch_create", "pano_batch_destroy", "pano_batch_set",
         "pano_batch_render", "pano_batch_set_timing", "pano_batch_stage_ms",
+        "pano_server_create", "pano_server_destroy", "pano_server_admit", "pano_server_retire",
+        "pano_server_submit", "pano_server_poll", "pano_server_drain", "pano_server_stats",
     ]
 
 
@@ -511,3 +522,82 @@ class Batch:
         _check(lib().pano_batch_render(self._h, _ptrs(ptrs), _sizes(steps), lp, _ptrs(optrs),
                                        _sizes(osteps), _stream(stream)), "batch render")
         return True
+
+
+class ServerDone(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("latency_ms", C.c_double), ("stream_id", C.c_int),
+                ("status", C.c_int), ("batch_frames", C.c_int), ("pad", C.c_int)]
+
+
+class Server:
+    """The serving loop (pano_server_*): prepared renders admitted as streams
+    against an HBM budget, frames batched continuously across streams.  The
+    tensors of a submitted frame are kept alive here until it is reported done."""
+
+    def __init__(self, hbm_budget_bytes: int, max_inflight: int = 2):
+        h = C.c_void_p()
+        _check(lib().pano_server_create(C.byref(h), int(hbm_budget_bytes), int(max_inflight)),
+               "server create")
+        self._h = h
+        self._keep = {}
+        self._renders = {}
+
+    def close(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            lib().pano_server_destroy(h)
+        self._h = None
+        self._keep.clear()
+
+    def __del__(self):
+        self.close()
+
+    def admit(self, render):
+        """Stream id, or None when the render does not fit the budget."""
+        sid = C.c_int()
+        st = lib().pano_server_admit(self._h, render._h, C.byref(sid))
+        if st == PANO_ERR_BUDGET:
+            return None
+        _check(st, "admit")
+        self._renders[sid.value] = render
+        return sid.value
+
+    def retire(self, sid: int):
+        _check(lib().pano_server_retire(self._h, int(sid)), "retire")
+        self._renders.pop(sid, None)
+
+    def submit(self, sid: int, srcs, out, luts=None, tag: int = 0):
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in srcs])
+        op, ostep = _dev_ptr_step(out)
+        lp = None
+        if luts is not None:
+            luts = np.ascontiguousarray(luts, np.uint8)
+            lp = C.c_void_p(luts.ctypes.data)
+        key = (int(sid), int(tag))
+        self._keep[key] = (list(srcs), out)
+        st = lib().pano_server_submit(self._h, int(sid), _ptrs(ptrs), _sizes(steps), lp,
+                                      C.c_void_p(op), ostep, C.c_uint64(int(tag)))
+        if st != PANO_OK:
+            self._keep.pop(key, None)
+        _check(st, "submit")
+
+    def poll(self, max_out: int = 256, wait_ms: int = 0):
+        """Completed frames: list of (stream_id, tag, latency_ms, status, batch_frames)."""
+        buf = (ServerDone * max_out)()
+        k = lib().pano_server_poll(self._h, buf, max_out, int(wait_ms))
+        if k < 0:
+            _check(k, "poll")
+        out = []
+        for d in buf[:k]:
+            self._keep.pop((d.stream_id, int(d.tag)), None)
+            out.append((d.stream_id, int(d.tag), d.latency_ms, d.status, d.batch_frames))
+        return out
+
+    def drain(self):
+        _check(lib().pano_server_drain(self._h), "drain")
+
+    def stats(self):
+        u, f, b = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().pano_server_stats(self._h, C.byref(u), C.byref(f), C.byref(b)), "stats")
+        return {"used_bytes": u.value, "frames_done": f.value, "batches": b.value}
+

@@ -694,20 +694,71 @@ struct pano_render {
     }
 };
 
+// A batch's launch tables live in one of kSlots device buffers.  A new
+// composition is uploaded (stream-ordered, from pinned memory) into the next
+// slot at its first render; a slot is reused only after the renders that read
+// it have finished, so changing the composition never waits for the device
+// unless kSlots compositions are still in flight.
 struct pano_batch {
+    static constexpr int kSlots = 4;
+    static constexpr size_t kSlotBytes = 64 << 10;   // >= 32 launches x 32 renders of Seg + 128 LutJob
+    struct Slot {
+        uint8_t* dev = nullptr;
+        hipEvent_t ev = nullptr;
+        bool used = false;
+    };
     std::vector<pano_render*> renders;
     int total_cams = 0;
-    Seg* d_segs = nullptr;
-    LutJob* d_lutjobs = nullptr;        // every render's LUT jobs, SrcTable indices of the batch
-    int n_lutjobs = 0, lut_blocks = 0;
-    std::vector<Launch> seq;
+    // host tables of the current composition: segs of every launch, then LUT jobs
+    std::vector<Seg> segs;
+    std::vector<LutJob> lutjobs;
+    std::vector<size_t> seg_off;         // per launch: first Seg
+    int lut_blocks = 0;
+    std::vector<Launch> seq;             // segs pointers into the current slot
+    bool dirty = false;                  // tables not yet uploaded
+    Slot slots[kSlots];
+    int cur = -1;
+    PinnedRing ring;
     StageTimer timer;
-    hipStream_t last = nullptr;
-    bool used = false;
     ~pano_batch()
     {
-        if (d_segs) (void)hipFree(d_segs);
-        if (d_lutjobs) (void)hipFree(d_lutjobs);
+        for (Slot& sl : slots) {
+            if (sl.used) (void)hipEventSynchronize(sl.ev);
+            if (sl.dev) (void)hipFree(sl.dev);
+            if (sl.ev) (void)hipEventDestroy(sl.ev);
+        }
+    }
+    int init()
+    {
+        for (Slot& sl : slots) {
+            if (hipMalloc((void**)&sl.dev, kSlotBytes) != hipSuccess) return PANO_ERR_NOMEM;
+            if (hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess) return PANO_ERR_HIP;
+        }
+        return ring.init(kSlotBytes);
+    }
+    // Upload the current tables into the next slot on stream s.
+    int upload(hipStream_t s)
+    {
+        const size_t sb = sizeof(Seg) * segs.size(), lb = sizeof(LutJob) * lutjobs.size();
+        const size_t loff = (sb + 255) / 256 * 256;
+        if (loff + lb > kSlotBytes) return PANO_ERR_UNSUPPORTED;
+        const int k = (cur + 1) % kSlots;
+        Slot& sl = slots[k];
+        if (sl.used) HIP_TRY(hipEventSynchronize(sl.ev));
+        std::vector<uint8_t> host(loff + lb);
+        if (sb) memcpy(host.data(), segs.data(), sb);
+        if (lb) memcpy(host.data() + loff, lutjobs.data(), lb);
+        int e = ring.upload(host.data(), host.size(), sl.dev, s);
+        if (e) return e;
+        for (size_t i = 0; i < seq.size(); i++) seq[i].segs = (const Seg*)sl.dev + seg_off[i];
+        cur = k;
+        dirty = false;
+        return PANO_OK;
+    }
+    const LutJob* lut_table() const
+    {
+        const size_t sb = sizeof(Seg) * segs.size();
+        return (const LutJob*)(slots[cur].dev + (sb + 255) / 256 * 256);
     }
 };
 
@@ -1185,8 +1236,15 @@ int64_t pano_render_model_bytes(const pano_render* r, int stage)
 int pano_batch_create(pano_batch** out)
 {
     if (!out) return PANO_ERR_ARG;
-    *out = new (std::nothrow) pano_batch();
-    return *out ? PANO_OK : PANO_ERR_NOMEM;
+    pano_batch* b = new (std::nothrow) pano_batch();
+    if (!b) return PANO_ERR_NOMEM;
+    int e = b->init();
+    if (e) {
+        delete b;
+        return e;
+    }
+    *out = b;
+    return PANO_OK;
 }
 
 void pano_batch_destroy(pano_batch* b) { delete b; }
@@ -1203,7 +1261,6 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
         maxL = std::max(maxL, r->rig->prep.levels);
     }
     if (cams > kMaxSrc) return PANO_ERR_UNSUPPORTED;
-    if (b->used) HIP_TRY(hipStreamSynchronize(b->last));
     // segment tables: one per launch, each render's share of the launch in order
     std::vector<std::vector<Seg>> tables;
     std::vector<Launch> ls;
@@ -1248,19 +1305,11 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
         j = d.d_fin;
         c = (int)d.prep.fin.size();
     });
-    size_t total = 0;
-    for (auto& t : tables) total += t.size();
     std::vector<Seg> flat;
-    flat.reserve(total);
-    for (auto& t : tables) flat.insert(flat.end(), t.begin(), t.end());
-    if (b->d_segs) HIP_TRY(hipFree(b->d_segs));
-    b->d_segs = nullptr;
-    if (hipMalloc((void**)&b->d_segs, sizeof(Seg) * flat.size()) != hipSuccess) return PANO_ERR_NOMEM;
-    HIP_TRY(hipMemcpy(b->d_segs, flat.data(), sizeof(Seg) * flat.size(), hipMemcpyHostToDevice));
-    size_t off = 0;
-    for (size_t k = 0; k < ls.size(); k++) {
-        ls[k].segs = b->d_segs + off;
-        off += tables[k].size();
+    std::vector<size_t> offs;
+    for (auto& t : tables) {
+        offs.push_back(flat.size());
+        flat.insert(flat.end(), t.begin(), t.end());
     }
     std::vector<LutJob> lj;
     int lblocks = 0, base = 0;
@@ -1274,16 +1323,16 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
         }
         base += d.prep.n;
     }
-    if (b->d_lutjobs) HIP_TRY(hipFree(b->d_lutjobs));
-    b->d_lutjobs = nullptr;
-    if (hipMalloc((void**)&b->d_lutjobs, sizeof(LutJob) * lj.size()) != hipSuccess)
-        return PANO_ERR_NOMEM;
-    HIP_TRY(hipMemcpy(b->d_lutjobs, lj.data(), sizeof(LutJob) * lj.size(), hipMemcpyHostToDevice));
-    b->n_lutjobs = (int)lj.size();
+    if (sizeof(Seg) * flat.size() + 256 + sizeof(LutJob) * lj.size() > pano_batch::kSlotBytes)
+        return PANO_ERR_UNSUPPORTED;
+    b->segs.swap(flat);
+    b->seg_off.swap(offs);
+    b->lutjobs.swap(lj);
     b->lut_blocks = lblocks;
     b->seq = ls;
     b->renders.assign(renders, renders + num);
     b->total_cams = cams;
+    b->dirty = true;
     return PANO_OK;
 }
 
@@ -1328,14 +1377,21 @@ int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* s
                 any = true;
             }
     }
-    b->last = stream;
-    b->used = true;
+    if (b->dirty) {
+        int e = b->upload(stream);
+        if (e) return e;
+    }
     LutPass lp;
-    lp.jobs = b->d_lutjobs;
-    lp.njobs = b->n_lutjobs;
+    lp.jobs = b->lut_table();
+    lp.njobs = (int)b->lutjobs.size();
     lp.blocks = b->lut_blocks;
     lp.src = st;
-    return run_sequence(b->seq, sr, ot, stream, b->timer.begin_frame(), any ? &lp : nullptr);
+    int e = run_sequence(b->seq, sr, ot, stream, b->timer.begin_frame(), any ? &lp : nullptr);
+    if (e) return e;
+    pano_batch::Slot& sl = b->slots[b->cur];
+    HIP_TRY(hipEventRecord(sl.ev, stream));
+    sl.used = true;
+    return PANO_OK;
 }
 
 int pano_batch_set_timing(pano_batch* b, int enable)

@@ -105,6 +105,30 @@ def workload(name: str, scale: float = 1.0, seed: int | None = None) -> Workload
     raise ValueError(name)
 
 
+SERVED_SIZES = [(1280, 720), (1920, 1080), (2560, 1440), (3840, 2160)]
+SERVED_OUTPUTS = [(2048, 1024, 5), (3840, 1920, 7), (7680, 3840, 9)]
+
+
+def served_rig(k: int) -> Workload:
+    """The k-th seeded rig of the served mix: 2-8 cameras of one size from
+    1280x720 to 3840x2160, fisheye (always for <= 3 cameras) or rectilinear
+    wide enough to overlap its neighbours, yaws evenly spaced, output
+    2048x1024 / 3840x1920 / 7680x3840 with 5 / 7 / 9 levels."""
+    rng = np.random.default_rng(SEED_BASE + 4000 + 7 * k)
+    n = int(rng.integers(2, 9))
+    w, h = SERVED_SIZES[int(rng.integers(0, len(SERVED_SIZES)))]
+    kind = "fisheye" if n <= 3 or rng.random() < 0.4 else "rect"
+    if kind == "fisheye":
+        w = h = min(w, h)
+        fov = float(rng.uniform(190, 200))
+    else:
+        fov = float(min(120.0, max(360.0 / n * 1.2, 60.0)))
+    cams = [Camera(kind, w, h, fov, 360.0 * i / n + float(rng.uniform(-3, 3)),
+                   pitch=float(rng.uniform(-3, 3))) for i in range(n)]
+    dw, dh, levels = SERVED_OUTPUTS[int(rng.integers(0, len(SERVED_OUTPUTS)))]
+    return Workload(f"served{k}", cams, dw, dh, levels, seed=SEED_BASE + 4000 + 7 * k)
+
+
 def value_noise(h: int, w: int, rng: np.random.Generator, octaves=(8, 32, 128)) -> np.ndarray:
     """Smooth noise in [0, 1): bilinear upsampling of random grids at several scales."""
     out = np.zeros((h, w), np.float32)

@@ -286,3 +286,69 @@ def test_batch_matches_alone_and_oracle():
     b.render([srcs[2], srcs[0]], o2, [luts[2], luts[0]])
     np.testing.assert_array_equal(from_dev(o2[0]), alone[2])
     np.testing.assert_array_equal(from_dev(o2[1]), alone[0])
+
+
+def test_server_results_same_alone_and_in_the_mix():
+    # Streams of different rigs and sizes through the serving loop: every frame
+    # equals the same frame rendered alone, and frames of several streams ran
+    # together in one batch.
+    cases = [("fisheye2", 0.25), ("rect6", 0.25), ("mixed8", 0.125), ("rect6", 0.2)]
+    rigs = []
+    for name, scale in cases:
+        wl = synth.workload(name, scale)
+        xs, ys, ms = synth.maps_and_masks(wl)
+        r = P.PanoramaRender()
+        assert r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h),
+                              xs, ys, ms, wl.levels)
+        frames = []
+        for f in range(3):
+            srcs = [to_dev(synth.bgrx(s)) for s in synth.frames(wl, f)]
+            lt = synth.luts(wl, f)
+            alone = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+            assert r.render(srcs, alone, lt)
+            frames.append((srcs, lt, from_dev(alone)))
+        rigs.append((wl, r, frames))
+    total = sum(r.mem_bytes() for _, r, _ in rigs)
+    # admission: the budget holds all rigs but the largest, which fits once the
+    # second largest leaves
+    by_size = sorted(range(len(rigs)), key=lambda k: rigs[k][1].mem_bytes())
+    big, second = by_size[-1], by_size[-2]
+    budget = total - rigs[second][1].mem_bytes()
+    srv = P.Server(budget, max_inflight=2)
+    ids = {}
+    for k, (_, r, _) in enumerate(rigs):
+        if k != big:
+            ids[k] = srv.admit(r)
+            assert ids[k] is not None
+    assert srv.admit(rigs[big][1]) is None
+    note("server: mix of 3 streams, then 4 after a retire")
+    outs = {}
+    rng = np.random.default_rng(3)
+    order = [(k, f) for f in range(3) for k in ids]
+    rng.shuffle(order)
+    for k, f in order:
+        wl, r, frames = rigs[k]
+        o = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+        outs[(k, f)] = o
+        srv.submit(ids[k], frames[f][0], o, frames[f][1], tag=f)
+    srv.drain()
+    done = srv.poll(64)
+    assert len(done) == len(order) and all(d[3] == 0 for d in done)
+    assert max(d[4] for d in done) > 1
+    # the second largest leaves and the largest is admitted in its place
+    srv.retire(ids.pop(second))
+    ids[big] = srv.admit(rigs[big][1])
+    assert ids[big] is not None and len(ids) == 3
+    for f in range(3):
+        for k in ids:
+            wl, r, frames = rigs[k]
+            o = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+            outs[(k, f, "b")] = o
+            srv.submit(ids[k], frames[f][0], o, frames[f][1], tag=10 + f)
+    srv.drain()
+    done = srv.poll(64)
+    assert len(done) == 3 * len(ids) and all(d[3] == 0 for d in done)
+    srv.close()
+    for key, o in outs.items():
+        k, f = key[0], key[1]
+        np.testing.assert_array_equal(from_dev(o), rigs[k][2][f][2])
