@@ -26,6 +26,7 @@ namespace pano {
 namespace {
 
 constexpr uint32_t kLo = 0x00ff00ffu;  // (B, R) lanes of a packed pixel; >> 8 gives (G, valid)
+constexpr uint32_t kHi = 0xff00ff00u;  // (G, valid) lanes in place
 
 // Down tile geometry: input region (2 kFTW + 3) x (2 kFTH + 3) at level j,
 // staged by kDT threads, kPer pixels each, with all loads of a thread issued
@@ -46,8 +47,8 @@ __device__ __forceinline__ void down_from_lds(const uint32_t* in, uint2* hs, con
         const uint32_t* p = in + r * kIW + 2 * x;
         uint32_t a = p[0], b = p[1], c = p[2], d = p[3], f = p[4];
         uint32_t lo = (a & kLo) + (f & kLo) + 4u * ((b & kLo) + (d & kLo)) + 6u * (c & kLo);
-        uint32_t hi = ((a >> 8) & kLo) + ((f >> 8) & kLo) +
-                      4u * (((b >> 8) & kLo) + ((d >> 8) & kLo)) + 6u * ((c >> 8) & kLo);
+        // (G, valid) in place: G sums reach bit 19, valid (0/1) sums sit at bits 24-28
+        uint32_t hi = ((a & kHi) + (f & kHi) + 4u * ((b & kHi) + (d & kHi)) + 6u * (c & kHi)) >> 8;
         hs[r * kFTW + x] = make_uint2(lo, hi);
     }
     __syncthreads();
@@ -187,6 +188,9 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
     const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
     const float fsw = (float)cam.sw, fsh = (float)cam.sh;
     const uint32_t owner_byte = (uint32_t)(J.src + 1) << 24;   // byte 3 of g0comp: owner + 1
+    // the tile's core (level-0 pixels 2X.., 2Y..) clipped to the canvas
+    const uint32_t core_w = (uint32_t)min(2 * J.nx, P0.cw - 2 * J.X);
+    const uint32_t core_h = (uint32_t)min(2 * J.ny, P0.ch - 2 * J.Y);
     // the staging region is one rectangle of the margined map storage
     const char* mbase = (const char*)(cam.map + ((size_t)J.my * cam.mpitch + J.mx));
     const uint32_t mrow = (uint32_t)cam.mpitch * 8u;
@@ -251,13 +255,11 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
             }
             v |= (by >> 31) << 24;   // kFlagValid
             // an owned core pixel (not a border-resolved copy of one) goes to the level-0 blend
-            if ((bx >> 31) && r >= 2 && c >= 2 && r < 2 + 2 * J.ny && c < 2 + 2 * J.nx) {
+            if ((bx >> 31) && (uint32_t)(c - 2) < core_w && (uint32_t)(r - 2) < core_h) {
                 const int vx = bx0 + c, vy = by0 + r;
-                if (vx < P0.cw && vy < P0.ch) {
-                    // canvas byte offsets fit 32 bits (checked at prepare)
-                    const uint32_t go = 4u * (__umul24((uint32_t)vy, (uint32_t)g0pitch) + (uint32_t)vx);
-                    stg((uint32_t*)((char*)g0comp + go), (v & 0xffffffu) | owner_byte);
-                }
+                // canvas byte offsets fit 32 bits (checked at prepare)
+                const uint32_t go = 4u * (__umul24((uint32_t)vy, (uint32_t)g0pitch) + (uint32_t)vx);
+                stg((uint32_t*)((char*)g0comp + go), (v & 0xffffffu) | owner_byte);
             }
             in[e] = v;
         }
@@ -462,15 +464,24 @@ __global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, 
     int lab[4];
     uint32_t g0[4];
 #pragma unroll
-    for (int d = 0; d < 4; d++) {
-        int Xp = X + (d & 1), Yp = Y + (d >> 1);
-        in[d] = Xp < J.X + J.nx && Yp < J.Y + J.ny;
-        // the owner's level-0 sample; byte 3 = owner + 1 (0: no owner, never written)
-        g0[d] = in[d] ? ldg((const uint32_t*)((const char*)R.g0comp +
-                                              4u * (__umul24((uint32_t)Yp, (uint32_t)R.g0pitch) + (uint32_t)Xp)))
-                      : 0u;
-        lab[d] = g0[d] >> 24 ? (int)(g0[d] >> 24) - 1 : (int)kNoOwner;
+    for (int d = 0; d < 4; d++) in[d] = X + (d & 1) < J.X + J.nx && Y + (d >> 1) < J.Y + J.ny;
+    // the owners' level-0 samples, a row pair at a time (X is even, the canvas
+    // pitch even: 8-byte aligned); byte 3 = owner + 1 (0: no owner, never written)
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const char* row = (const char*)R.g0comp + 4u * (__umul24((uint32_t)(Y + h), (uint32_t)R.g0pitch) +
+                                                        (uint32_t)X);
+        if (in[2 * h + 1]) {
+            const uint2 v = ldg_u2((const uint2*)row);
+            g0[2 * h] = v.x;
+            g0[2 * h + 1] = v.y;
+        } else {
+            g0[2 * h] = in[2 * h] ? ldg((const uint32_t*)row) : 0u;
+            g0[2 * h + 1] = 0u;
+        }
     }
+#pragma unroll
+    for (int d = 0; d < 4; d++) lab[d] = g0[d] >> 24 ? (int)(g0[d] >> 24) - 1 : (int)kNoOwner;
     const CanvasLevel C1 = ldc(R.canvas + (L > 0 ? 1 : 0));
     const StageCoords SC = stage_coords(sx0, sy0, C1.cw, C1.ch);
     if (L > 0) {
@@ -531,15 +542,24 @@ __global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, 
     }
     uint8_t* out = outs.p[Sg.out];
     const uint32_t ostep = outs.step[Sg.out];
+    const bool pairs = ((ostep | (uint32_t)(uintptr_t)out) & 7u) == 0;   // 8-byte aligned rows
+    uint32_t o[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) {
-        if (!in[d]) continue;
-        int Xp = X + (d & 1), Yp = Y + (d >> 1);
-        uint32_t o = 0;
+        o[d] = 0;
         if (R.full || lab[d] != kNoOwner)
-            o = (uint32_t)sat_u8(up[d].x + lap[d][0]) | ((uint32_t)sat_u8(up[d].y + lap[d][1]) << 8) |
-                ((uint32_t)sat_u8(up[d].z + lap[d][2]) << 16);
-        stg((uint32_t*)(out + (__umul24((uint32_t)Yp, ostep) + 4u * (uint32_t)Xp)), o);
+            o[d] = (uint32_t)sat_u8(up[d].x + lap[d][0]) | ((uint32_t)sat_u8(up[d].y + lap[d][1]) << 8) |
+                   ((uint32_t)sat_u8(up[d].z + lap[d][2]) << 16);
+    }
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        uint32_t* row = (uint32_t*)(out + (__umul24((uint32_t)(Y + h), ostep) + 4u * (uint32_t)X));
+        if (pairs && in[2 * h + 1]) {
+            stg_u2((uint2*)row, make_uint2(o[2 * h], o[2 * h + 1]));
+        } else {
+            if (in[2 * h]) stg(row, o[2 * h]);
+            if (in[2 * h + 1]) stg(row + 1, o[2 * h + 1]);
+        }
     }
 }
 

@@ -59,6 +59,16 @@ __device__ __forceinline__ short4 ldg_s4(const short4* p)
     return make_short4(v.x, v.y, v.z, v.w);
 }
 typedef unsigned int pano_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint2 ldg_u2(const uint2* p)
+{
+    pano_u32x2 v = *(const PANO_GLOBAL pano_u32x2*)p;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ void stg_u2(uint2* p, uint2 v)
+{
+    pano_u32x2 w = {v.x, v.y};
+    *(PANO_GLOBAL pano_u32x2*)p = w;
+}
 // C_j pixel of the 8-bit path: channels 0..2 as int16.
 __device__ __forceinline__ int4 ldg_c16(const uint2* p)
 {
@@ -280,7 +290,8 @@ __device__ __forceinline__ uint32_t bilinear_fast(float wx, float wy, const uint
         float v = __fmaf_rn(h1 - h0, wy, h0);
         float f = __builtin_amdgcn_fractf(v) - 0.5f;
         tie |= __builtin_fabsf(f) < kTieEps;
-        out |= (uint32_t)(int)__builtin_rintf(v) << sh8;
+        // round to nearest and saturate into byte c (v is within 2^-15 of [0, 255])
+        out = __builtin_amdgcn_cvt_pk_u8_f32(v, (uint32_t)c, out);
     }
     return out;
 }
