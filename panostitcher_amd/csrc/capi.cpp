@@ -229,6 +229,7 @@ struct DeviceRig {
     uint8_t* d_masks = nullptr;             // full-canvas masks (caller-image level 0)
     uint8_t* d_luts = nullptr;
     int* d_fin_owners = nullptr;
+    PlaneRef* d_fin_refs = nullptr;
     uint32_t* d_g0comp = nullptr;           // owner's level-0 value per canvas pixel (8-bit render)
     Seg* d_segs = nullptr;                  // render() alone: one segment per fused launch
     std::vector<Launch> seq;                // render() alone: the fused launch sequence
@@ -271,6 +272,7 @@ struct DeviceRig {
             pl.add(sizeof(double) * (3 * n + 1));
         }
         pl.add(sizeof(int) * prep.fin_owners.size());
+        if (fused) pl.add(sizeof(PlaneRef) * prep.fin_owners.size());
         for (int j = 1; j <= L; j++) {
             pl.add((size_t)prep.lw[j] * prep.lh[j] * (fused ? 8 : 16));
             if (!prep.full) pl.add((size_t)prep.lw[j] * prep.lh[j] * 2);
@@ -390,6 +392,23 @@ struct DeviceRig {
         d_camlist = upload_vec(arena, prep.camlist, err);
         d_fin = upload_vec(arena, prep.fin, err);
         d_fin_owners = upload_vec(arena, prep.fin_owners, err);
+        if (fused && L > 0) {
+            std::vector<PlaneRef> refs(prep.fin_owners.size());
+            for (size_t k = 0; k < refs.size(); k++) {
+                const int o = prep.fin_owners[k];
+                const Plane& p1 = planes[(size_t)o * (L + 1) + 1];
+                PlaneRef& f = refs[k];
+                f.g = p1.g;
+                f.x0 = p1.x0;
+                f.y0 = p1.y0;
+                f.bw = p1.bw;
+                f.bh = p1.bh;
+                f.cw = p1.cw;
+                f.pitch = p1.pitch;
+                f.cam = o;
+            }
+            d_fin_refs = upload_vec(arena, refs, err);
+        }
         if (fused) {
             d_g0comp = (uint32_t*)arena.take(P * 4);
             if (!d_g0comp) return PANO_ERR_NOMEM;
@@ -500,6 +519,7 @@ struct DeviceRig {
         rd.canvas = d_canvas;
         rd.camlist = d_camlist;
         rd.owners = d_fin_owners;
+        rd.fin_refs = d_fin_refs;
         rd.cams = d_cams;
         rd.luts = d_luts;
         rd.g0comp = d_g0comp;

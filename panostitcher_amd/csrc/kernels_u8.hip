@@ -79,7 +79,8 @@ __device__ __forceinline__ uint32_t load_u32(const Plane& P, int X, int Y)
 
 // Pixel (X, Y) of a plane's box as a byte offset, or -1 outside it (box
 // pitches and rows stay below 2^24, the plane below 2^31 bytes).
-__device__ __forceinline__ int box_off32(const Plane& P, int X, int Y)
+template <class Pl>
+__device__ __forceinline__ int box_off32(const Pl& P, int X, int Y)
 {
     const int ly = Y - P.y0;
     int lx = X - P.x0;
@@ -88,7 +89,8 @@ __device__ __forceinline__ int box_off32(const Plane& P, int X, int Y)
     return (int)(4u * (__umul24((uint32_t)ly, (uint32_t)P.pitch) + (uint32_t)lx));
 }
 
-__device__ __forceinline__ uint32_t load_u32_off(const Plane& P, int off)
+template <class Pl>
+__device__ __forceinline__ uint32_t load_u32_off(const Pl& P, int off)
 {
     return off < 0 ? 0u : ldg((const uint32_t*)((const char*)P.g + off));
 }
@@ -446,16 +448,16 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
 
 // ------------------------------------------------------------ level 0 --
 
+constexpr int kFinStage = 4;   // owner cameras staged at once (a tile rarely has more than 2)
+
 __global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, int nseg,
                                                   OutTable outs)
 {
-    __shared__ uint32_t gs[kMaxStage][kSH * kSW];
+    __shared__ uint32_t gs[kFinStage][kSH * kSW];
     __shared__ int4 cs[kSH * kSW];
     const Seg& Sg = seg_of(segs, nseg, blockIdx.x);
     const RenderDesc R = ldc(Sg.rd);
     const TileJob J = ldc((const TileJob*)Sg.jobs + (blockIdx.x - Sg.begin));
-    const Plane* planes = R.planes;
-    const int* owners = R.owners;
     const int L = R.levels;
     const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
     const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
@@ -505,28 +507,31 @@ __global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, 
 #pragma unroll
     for (int d = 0; d < 4; d++) up[d] = make_int3(0, 0, 0);
     if (L > 0) {
-        for (int g = 0; g < J.cnt; g += kMaxStage) {
-            const int ng = J.cnt - g < kMaxStage ? J.cnt - g : kMaxStage;
-            uint32_t gv[kMaxStage][kStageIt];
+        for (int g = 0; g < J.cnt; g += kFinStage) {
+            const int ng = J.cnt - g < kFinStage ? J.cnt - g : kFinStage;
+            // the group's owner planes: independent descriptor loads
+            PlaneRef O[kFinStage];
 #pragma unroll
-            for (int s = 0; s < kMaxStage; s++) {
-                const int o = ldc(owners + J.off + g + (s < ng ? s : 0));
-                const Plane P1 = ldc(planes + o * (L + 1) + 1);
+            for (int s = 0; s < kFinStage; s++) O[s] = ldc(R.fin_refs + J.off + g + (s < ng ? s : 0));
+            uint32_t gv[kFinStage][kStageIt];
+#pragma unroll
+            for (int s = 0; s < kFinStage; s++)
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++)
-                    gv[s][k] = (s < ng && SC.ok[k]) ? load_u32_off(P1, box_off32(P1, SC.xs[k], SC.ys[k])) : 0u;
-            }
+                    gv[s][k] = (s < ng && SC.ok[k]) ? load_u32_off(O[s], box_off32(O[s], SC.xs[k], SC.ys[k]))
+                                                    : 0u;
 #pragma unroll
-            for (int s = 0; s < kMaxStage; s++)
+            for (int s = 0; s < kFinStage; s++)
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++) {
                     int e = threadIdx.x + k * 256;
                     if (s < ng && SC.ok[k]) gs[s][e] = gv[s][k];
                 }
             __syncthreads();
-            for (int s = 0; s < ng; s++) {
-                const int o = ldc(owners + J.off + g + s);
-                if (lab[0] != o && lab[1] != o && lab[2] != o && lab[3] != o) continue;
+#pragma unroll
+            for (int s = 0; s < kFinStage; s++) {
+                const int o = O[s].cam;
+                if (s >= ng || (lab[0] != o && lab[1] != o && lab[2] != o && lab[3] != o)) continue;
                 uint32_t u[4];
                 up_quad_u8(&gs[s][(qy + 1) * kSW + qx + 1], kSW, u);
 #pragma unroll

@@ -63,6 +63,16 @@ struct CanvasLevel {
 
 struct RemapCam;
 
+// A camera's plane as one tile of a stage reads it: the owner cameras of a
+// level-0 tile (their level-1 planes), built at prepare so that each is one
+// independent descriptor load.
+struct PlaneRef {
+    const void* g;
+    int x0, y0, bw, bh, cw, pitch;
+    int cam;       // camera index (the owner id of level-0 pixels)
+    int pad;
+};
+
 // Per render (rig), in device memory.  The generic 16-bit kernels index the
 // launch's arrays with plane0 / canvas0 / out; the 8-bit kernels reach every
 // array of their render through the pointers.
@@ -72,6 +82,7 @@ struct RenderDesc {
     const CanvasLevel* canvas;   // levels + 1
     const int* camlist;          // blend cameras (plane indices)
     const int* owners;           // level-0 owner cameras
+    const PlaneRef* fin_refs;    // level-1 planes of the owners of each level-0 tile
     const RemapCam* cams;        // remap inputs
     const uint8_t* luts;         // n x 768 LUT bytes, then 1 byte: LUTs active this frame
     uint32_t* g0comp;            // owners' level-0 samples; byte 3 = owner + 1, 0 = no owner
