@@ -138,6 +138,21 @@ int host_reflect(int p, int n)
 int map_pitch(const Geom& g) { return g.bw > 0 ? (g.bw + 2 * kMapMargin + 63) / 64 * 64 : 0; }
 size_t map_cells(const Geom& g) { return g.bh > 0 ? (size_t)map_pitch(g) * (g.bh + 2 * kMapMargin) : 0; }
 
+PlaneRef plane_ref(const Plane& p, int cam)
+{
+    PlaneRef f{};
+    f.g = p.g;
+    f.w = p.w;
+    f.x0 = p.x0;
+    f.y0 = p.y0;
+    f.bw = p.bw;
+    f.bh = p.bh;
+    f.cw = p.cw;
+    f.pitch = p.pitch;
+    f.cam = cam;
+    return f;
+}
+
 // Row step of a LUT-mapped frame copy.
 size_t lut_step(int w) { return ((size_t)w * 4 + 255) / 256 * 256; }
 
@@ -230,6 +245,8 @@ struct DeviceRig {
     uint8_t* d_luts = nullptr;
     int* d_fin_owners = nullptr;
     PlaneRef* d_fin_refs = nullptr;
+    PlaneRef* d_blend_refs = nullptr;
+    PlaneRef* d_blend_refs1 = nullptr;
     uint32_t* d_g0comp = nullptr;           // owner's level-0 value per canvas pixel (8-bit render)
     Seg* d_segs = nullptr;                  // render() alone: one segment per fused launch
     std::vector<Launch> seq;                // render() alone: the fused launch sequence
@@ -273,6 +290,10 @@ struct DeviceRig {
         }
         pl.add(sizeof(int) * prep.fin_owners.size());
         if (fused) pl.add(sizeof(PlaneRef) * prep.fin_owners.size());
+        if (fused) {
+            pl.add(sizeof(PlaneRef) * prep.camlist.size());
+            pl.add(sizeof(PlaneRef) * prep.camlist.size());
+        }
         for (int j = 1; j <= L; j++) {
             pl.add((size_t)prep.lw[j] * prep.lh[j] * (fused ? 8 : 16));
             if (!prep.full) pl.add((size_t)prep.lw[j] * prep.lh[j] * 2);
@@ -396,18 +417,18 @@ struct DeviceRig {
             std::vector<PlaneRef> refs(prep.fin_owners.size());
             for (size_t k = 0; k < refs.size(); k++) {
                 const int o = prep.fin_owners[k];
-                const Plane& p1 = planes[(size_t)o * (L + 1) + 1];
-                PlaneRef& f = refs[k];
-                f.g = p1.g;
-                f.x0 = p1.x0;
-                f.y0 = p1.y0;
-                f.bw = p1.bw;
-                f.bh = p1.bh;
-                f.cw = p1.cw;
-                f.pitch = p1.pitch;
-                f.cam = o;
+                refs[k] = plane_ref(planes[(size_t)o * (L + 1) + 1], o);
             }
             d_fin_refs = upload_vec(arena, refs, err);
+            // blend tiles: camlist entries are plane indices cam (L + 1) + j
+            std::vector<PlaneRef> r0(prep.camlist.size()), r1(prep.camlist.size());
+            for (size_t k = 0; k < r0.size(); k++) {
+                const int p = prep.camlist[k], cam = p / (L + 1), j = p % (L + 1);
+                r0[k] = plane_ref(planes[p], cam);
+                r1[k] = j < L ? plane_ref(planes[p + 1], cam) : r0[k];
+            }
+            d_blend_refs = upload_vec(arena, r0, err);
+            d_blend_refs1 = upload_vec(arena, r1, err);
         }
         if (fused) {
             d_g0comp = (uint32_t*)arena.take(P * 4);
@@ -520,6 +541,8 @@ struct DeviceRig {
         rd.camlist = d_camlist;
         rd.owners = d_fin_owners;
         rd.fin_refs = d_fin_refs;
+        rd.blend_refs = d_blend_refs;
+        rd.blend_refs1 = d_blend_refs1;
         rd.cams = d_cams;
         rd.luts = d_luts;
         rd.g0comp = d_g0comp;

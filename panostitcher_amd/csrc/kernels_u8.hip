@@ -338,15 +338,15 @@ __global__ __launch_bounds__(kDT) void k_down_u8(const Seg* __restrict__ segs, i
 
 // ------------------------------------------------------- blend, level >= 1 --
 
+constexpr int kBlendStage = 4;   // cameras of a blend tile staged at once
+
 __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, int nseg, int level)
 {
-    __shared__ uint32_t gs[kMaxStage][kSH * kSW];
+    __shared__ uint32_t gs[kBlendStage][kSH * kSW];
     __shared__ int4 cs[kSH * kSW];
     const Seg& Sg = seg_of(segs, nseg, blockIdx.x);
     const RenderDesc R = ldc(Sg.rd);
     const TileJob J = ldc((const TileJob*)Sg.jobs + (blockIdx.x - Sg.begin));
-    const Plane* planes = R.planes;
-    const int* camlist = R.camlist;
     const bool top = J.flags & kTileTop;
     const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
     const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
 #pragma unroll
         for (int d = 0; d < 4; d++)
             if (in[d]) ws[d] = ldg(C.wsum + (__umul24((uint32_t)(Y + (d >> 1)), (uint32_t)C.pitch) + X + (d & 1)));
-    // stage up to kMaxStage cameras' level j+1 regions and C_{j+1} with every load in flight
+    // stage C_{j+1} and the group's level j+1 regions with every load in flight
     const CanvasLevel C1 = ldc(R.canvas + (top ? level : level + 1));
     const StageCoords SC = stage_coords(sx0, sy0, C1.cw, C1.ch);
     if (!top) {
@@ -378,19 +378,21 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
     int3 acc[4];
 #pragma unroll
     for (int d = 0; d < 4; d++) acc[d] = make_int3(0, 0, 0);
-    for (int g = 0; g < J.cnt; g += kMaxStage) {
-        const int ng = J.cnt - g < kMaxStage ? J.cnt - g : kMaxStage;
+    for (int g = 0; g < J.cnt; g += kBlendStage) {
+        const int ng = J.cnt - g < kBlendStage ? J.cnt - g : kBlendStage;
         if (!top) {
-            uint32_t gv[kMaxStage][kStageIt];
+            PlaneRef P1[kBlendStage];
 #pragma unroll
-            for (int s = 0; s < kMaxStage; s++) {
-                const Plane P1 = ldc(planes + ldc(camlist + J.off + g + (s < ng ? s : 0)) + 1);
+            for (int s = 0; s < kBlendStage; s++) P1[s] = ldc(R.blend_refs1 + J.off + g + (s < ng ? s : 0));
+            uint32_t gv[kBlendStage][kStageIt];
+#pragma unroll
+            for (int s = 0; s < kBlendStage; s++)
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++)
-                    gv[s][k] = (s < ng && SC.ok[k]) ? load_u32_off(P1, box_off32(P1, SC.xs[k], SC.ys[k])) : 0u;
-            }
+                    gv[s][k] = (s < ng && SC.ok[k]) ? load_u32_off(P1[s], box_off32(P1[s], SC.xs[k], SC.ys[k]))
+                                                    : 0u;
 #pragma unroll
-            for (int s = 0; s < kMaxStage; s++)
+            for (int s = 0; s < kBlendStage; s++)
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++) {
                     int e = threadIdx.x + k * 256;
@@ -398,15 +400,21 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
                 }
         }
         __syncthreads();
-        for (int s = 0; s < ng; s++) {
-            const Plane P = ldc(planes + ldc(camlist + J.off + g + s));
+        // this level's pixels and weights of the group's cameras, one camera at a time
+        // (holding all of them in registers costs more occupancy than it saves)
+        PlaneRef P[kBlendStage];
+#pragma unroll
+        for (int s = 0; s < kBlendStage; s++) P[s] = ldc(R.blend_refs + J.off + g + (s < ng ? s : 0));
+#pragma unroll
+        for (int s = 0; s < kBlendStage; s++) {
+            if (s >= ng) continue;
             int w[4];
             uint32_t gp[4];
 #pragma unroll
             for (int d = 0; d < 4; d++) {
-                const int off = in[d] ? box_off32(P, X + (d & 1), Y + (d >> 1)) : -1;
-                w[d] = off >= 0 ? (int)ldg((const uint16_t*)((const char*)P.w + (off >> 1))) : 0;
-                gp[d] = load_u32_off(P, off);
+                const int off = in[d] ? box_off32(P[s], X + (d & 1), Y + (d >> 1)) : -1;
+                w[d] = off >= 0 ? (int)ldg((const uint16_t*)((const char*)P[s].w + (off >> 1))) : 0;
+                gp[d] = load_u32_off(P[s], off);
             }
             uint32_t u[4] = {0, 0, 0, 0};
             if (!top) up_quad_u8(&gs[s][(qy + 1) * kSW + qx + 1], kSW, u);

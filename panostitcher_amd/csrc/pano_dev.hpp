@@ -63,13 +63,14 @@ struct CanvasLevel {
 
 struct RemapCam;
 
-// A camera's plane as one tile of a stage reads it: the owner cameras of a
-// level-0 tile (their level-1 planes), built at prepare so that each is one
-// independent descriptor load.
+// A camera's plane as one tile of a stage reads it (the owner cameras of a
+// level-0 tile, the cameras of a blend tile), built at prepare so that each
+// is one independent descriptor load.
 struct PlaneRef {
     const void* g;
+    const uint16_t* w;   // weights (blend tiles' level-j planes)
     int x0, y0, bw, bh, cw, pitch;
-    int cam;       // camera index (the owner id of level-0 pixels)
+    int cam;             // camera index (the owner id of level-0 pixels)
     int pad;
 };
 
@@ -83,6 +84,8 @@ struct RenderDesc {
     const int* camlist;          // blend cameras (plane indices)
     const int* owners;           // level-0 owner cameras
     const PlaneRef* fin_refs;    // level-1 planes of the owners of each level-0 tile
+    const PlaneRef* blend_refs;  // per camlist entry: the camera's level-j plane
+    const PlaneRef* blend_refs1; // per camlist entry: its level-(j+1) plane (j < L)
     const RemapCam* cams;        // remap inputs
     const uint8_t* luts;         // n x 768 LUT bytes, then 1 byte: LUTs active this frame
     uint32_t* g0comp;            // owners' level-0 samples; byte 3 = owner + 1, 0 = no owner
