@@ -8,9 +8,10 @@ equirect canvas with 9 pyramid levels.  A step stitches one frame of each of S
 streams (default 16) with one launch per pyramid stage (pano_batch_render);
 inputs (camera frames, maps, seams, weights) are resident in HBM before timing
 starts and the timed region is K back-to-back steps on one stream, bracketed by
-device synchronisation.  value = output Mpix/s over all streams.  The same line
-reports one stream alone (render()) and the plain drop-in (reproject + blender,
-the reference's GPU API usage).
+device synchronisation.  value = output Mpix/s over all streams, from the faster
+of the eager batched step and its replay as a captured HIP graph (both are in
+the line).  The same line reports one stream alone (render()) and the plain
+drop-in (reproject + blender, the reference's GPU API usage).
 
 Other configurations of the same path (one JSON line each, same fields):
   c1  2 fisheyes 1920x1920 195 deg -> 2048x1024, L=5, one frame per step
@@ -286,6 +287,21 @@ def run_batched(args):
         lat.append(sum(st.values()))
     batch.set_timing(False)
     log(f"batched {S} streams: {args.steps} steps in {elapsed * 1e3:.1f} ms; per step {per_step}")
+    eager = {"value": round(S * wl.out_pixels * args.steps / elapsed / 1e6, 2), "unit": "Mpix/s",
+             "ms_per_step": round(elapsed * 1e3 / args.steps, 4)}
+
+    # the same step captured as a HIP graph and replayed (frames refreshed in place)
+    graph = None
+    if not args.no_graph:
+        side = torch.cuda.Stream()   # a capturable (non-null) stream, default priority
+        torch.cuda.synchronize()
+        batch.capture(stream_src, outs, lut_list, side)
+        dtg = timed(lambda: batch.replay(lut_list, side), args.steps, args.warmup)
+        graph = {"value": round(S * wl.out_pixels * args.steps / dtg / 1e6, 2), "unit": "Mpix/s",
+                 "ms_per_step": round(dtg * 1e3 / args.steps, 4)}
+        log(f"graph replay {graph}")
+        if dtg < elapsed:
+            elapsed = dtg
 
     ms_per_step = elapsed * 1e3 / args.steps
     value = S * wl.out_pixels * args.steps / elapsed / 1e6
@@ -316,6 +332,8 @@ def run_batched(args):
                           "streams": S,
                           "note": "device time of a step; each stream's frame completes with it"}
     line["single_stream"] = single
+    line["batched_eager"] = eager
+    line["batched_graph"] = graph
     line["plain_dropin"] = plain
     if plain:
         line["vs_plain_dropin"] = round(value / plain["value"], 3)
@@ -598,6 +616,7 @@ def main():
     ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()

@@ -222,6 +222,18 @@ int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* s
                       hipStream_t stream);
 int pano_batch_set_timing(pano_batch* b, int enable);
 int pano_batch_stage_ms(pano_batch* b, double* ms4, int* frames);
+/* The per-frame sequence of one pano_batch_render as a HIP graph (no
+ * counterpart in the reference): capture records the launches for these
+ * frame and canvas buffers (and uploads LUTs and tables outside the graph);
+ * each replay re-runs them on the stream, for frames refreshed in place.
+ * Renders captured with LUTs take new LUT contents at replay (luts[i], or NULL
+ * to keep the previous ones); the others must pass NULL.  pano_batch_set drops
+ * the graph.  With timing enabled, a replay's time is reported as stage 0.
+ * The stream of a capture must not be the null stream. */
+int pano_batch_capture(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                       const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                       hipStream_t stream);
+int pano_batch_replay(pano_batch* b, const uint8_t* const* luts, hipStream_t stream);
 
 /* ---- serving loop ----------------------------------------------------------
  * Replaces the per-stream worker threads of the live task

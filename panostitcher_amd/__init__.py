@@ -109,6 +109,8 @@ def lib():
             "pano_batch_render": ([vp, pp, psz, pp, pp, psz, vp], i),
             "pano_batch_set_timing": ([vp, i], i),
             "pano_batch_stage_ms": ([vp, vp, C.POINTER(C.c_int)], i),
+            "pano_batch_capture": ([vp, pp, psz, pp, pp, psz, vp], i),
+            "pano_batch_replay": ([vp, pp, vp], i),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
@@ -135,6 +137,7 @@ def exported_symbols():
         "pano_render_fused", "pano_render_render_autogain", "pano_render_gains",
         "pano_render_gain_sums", "pano_batch_create", "pano_batch_destroy", "pano_batch_set",
         "pano_batch_render", "pano_batch_set_timing", "pano_batch_stage_ms",
+        "pano_batch_capture", "pano_batch_replay",
         "pano_server_create", "pano_server_destroy", "pano_server_admit", "pano_server_retire",
         "pano_server_submit", "pano_server_poll", "pano_server_drain", "pano_server_stats",
     ]
@@ -501,6 +504,20 @@ class Batch:
         _check(lib().pano_batch_stage_ms(self._h, ms.ctypes.data, C.byref(nf)), "stage times")
         return dict(zip(self.STAGES, ms.tolist())), nf.value
 
+    @staticmethod
+    def _lut_array(luts_per_render, keep):
+        if luts_per_render is None:
+            return None
+        arr = []
+        for lt in luts_per_render:
+            if lt is None:
+                arr.append(None)
+            else:
+                a = np.ascontiguousarray(lt, np.uint8)
+                keep.append(a)
+                arr.append(a.ctypes.data)
+        return (C.c_void_p * len(arr))(*[C.c_void_p(a) if a else None for a in arr])
+
     def render(self, srcs_per_render, outs, luts_per_render=None, stream=None):
         """srcs_per_render: list (per render) of lists of BGRx cuda tensors;
         outs: per render BGRx cuda tensor; luts_per_render: per render (n,3,256) or None."""
@@ -508,20 +525,27 @@ class Batch:
         ptrs, steps = zip(*[_dev_ptr_step(t) for t in flat])
         optrs, osteps = zip(*[_dev_ptr_step(t) for t in outs])
         keep = []
-        lp = None
-        if luts_per_render is not None:
-            arr = []
-            for lt in luts_per_render:
-                if lt is None:
-                    arr.append(None)
-                else:
-                    a = np.ascontiguousarray(lt, np.uint8)
-                    keep.append(a)
-                    arr.append(a.ctypes.data)
-            lp = (C.c_void_p * len(arr))(*[C.c_void_p(a) if a else None for a in arr])
+        lp = self._lut_array(luts_per_render, keep)
         _check(lib().pano_batch_render(self._h, _ptrs(ptrs), _sizes(steps), lp, _ptrs(optrs),
                                        _sizes(osteps), _stream(stream)), "batch render")
         return True
+
+    def capture(self, srcs_per_render, outs, luts_per_render=None, stream=None):
+        """Records one render of these buffers as a HIP graph (pano_batch_capture);
+        the tensors are kept alive for the replays."""
+        flat = [t for srcs in srcs_per_render for t in srcs]
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in flat])
+        optrs, osteps = zip(*[_dev_ptr_step(t) for t in outs])
+        keep = []
+        lp = self._lut_array(luts_per_render, keep)
+        _check(lib().pano_batch_capture(self._h, _ptrs(ptrs), _sizes(steps), lp, _ptrs(optrs),
+                                        _sizes(osteps), _stream(stream)), "batch capture")
+        self._graph_buffers = (flat, list(outs))
+
+    def replay(self, luts_per_render=None, stream=None):
+        keep = []
+        lp = self._lut_array(luts_per_render, keep)
+        _check(lib().pano_batch_replay(self._h, lp, _stream(stream)), "batch replay")
 
 
 class ServerDone(C.Structure):

@@ -763,8 +763,21 @@ struct pano_batch {
     int cur = -1;
     PinnedRing ring;
     StageTimer timer;
+    // captured launch sequence (pano_batch_capture) and the tables it reads
+    hipGraphExec_t exec = nullptr;
+    int exec_slot = -1;
+    std::vector<char> exec_luts;          // per render: captured with the LUT pass
+    void drop_graph()
+    {
+        if (!exec) return;
+        if (exec_slot >= 0 && slots[exec_slot].used) (void)hipEventSynchronize(slots[exec_slot].ev);
+        (void)hipGraphExecDestroy(exec);
+        exec = nullptr;
+        exec_slot = -1;
+    }
     ~pano_batch()
     {
+        drop_graph();
         for (Slot& sl : slots) {
             if (sl.used) (void)hipEventSynchronize(sl.ev);
             if (sl.dev) (void)hipFree(sl.dev);
@@ -1368,6 +1381,7 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
     }
     if (sizeof(Seg) * flat.size() + 256 + sizeof(LutJob) * lj.size() > pano_batch::kSlotBytes)
         return PANO_ERR_UNSUPPORTED;
+    b->drop_graph();
     b->segs.swap(flat);
     b->seg_off.swap(offs);
     b->lutjobs.swap(lj);
@@ -1379,15 +1393,22 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
     return PANO_OK;
 }
 
-int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
-                      const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
-                      hipStream_t stream)
+}  // extern "C"
+
+namespace {
+
+// The frame and canvas tables of one batched render, after checking the
+// arguments; uploads the renders' LUTs and the batch's tables on the stream.
+// sr: the frames the remap reads (LUT-mapped copies for renders with LUTs).
+int batch_prepare(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                  const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                  hipStream_t stream, SrcTable& st, SrcTable& sr, OutTable& ot, bool& any)
 {
     if (!b) return PANO_ERR_ARG;
     if (b->renders.empty()) return PANO_ERR_STATE;
     if (!srcs || !src_steps || !dsts || !dst_steps) return PANO_ERR_ARG;
-    SrcTable st{};
-    OutTable ot{};
+    st = SrcTable{};
+    ot = OutTable{};
     int k = 0;
     for (size_t i = 0; i < b->renders.size(); i++) {
         pano_render* r = b->renders[i];
@@ -1405,8 +1426,8 @@ int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* s
         }
     }
     // frames of renders with LUTs this call are read from their LUT-mapped copies
-    SrcTable sr = st;
-    bool any = false;
+    sr = st;
+    any = false;
     k = 0;
     for (size_t i = 0; i < b->renders.size(); i++) {
         pano_render* r = b->renders[i];
@@ -1424,13 +1445,88 @@ int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* s
         int e = b->upload(stream);
         if (e) return e;
     }
+    return PANO_OK;
+}
+
+LutPass batch_lut_pass(const pano_batch* b, const SrcTable& st)
+{
     LutPass lp;
     lp.jobs = b->lut_table();
     lp.njobs = (int)b->lutjobs.size();
     lp.blocks = b->lut_blocks;
     lp.src = st;
-    int e = run_sequence(b->seq, sr, ot, stream, b->timer.begin_frame(), any ? &lp : nullptr);
+    return lp;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                      const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                      hipStream_t stream)
+{
+    SrcTable st, sr;
+    OutTable ot;
+    bool any = false;
+    int e = batch_prepare(b, srcs, src_steps, luts, dsts, dst_steps, stream, st, sr, ot, any);
     if (e) return e;
+    const LutPass lp = batch_lut_pass(b, st);
+    e = run_sequence(b->seq, sr, ot, stream, b->timer.begin_frame(), any ? &lp : nullptr);
+    if (e) return e;
+    pano_batch::Slot& sl = b->slots[b->cur];
+    HIP_TRY(hipEventRecord(sl.ev, stream));
+    sl.used = true;
+    return PANO_OK;
+}
+
+int pano_batch_capture(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                       const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                       hipStream_t stream)
+{
+    if (!b || !stream) return PANO_ERR_ARG;   // the null stream cannot be captured
+    b->drop_graph();
+    SrcTable st, sr;
+    OutTable ot;
+    bool any = false;
+    // uploads happen here, outside the capture: the graph holds kernel launches only
+    int e = batch_prepare(b, srcs, src_steps, luts, dsts, dst_steps, stream, st, sr, ot, any);
+    if (e) return e;
+    const LutPass lp = batch_lut_pass(b, st);
+    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    e = run_sequence(b->seq, sr, ot, stream, nullptr, any ? &lp : nullptr);
+    hipGraph_t graph = nullptr;
+    const hipError_t ce = hipStreamEndCapture(stream, &graph);
+    if (!e && ce != hipSuccess) e = PANO_ERR_HIP;
+    if (!e && hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        b->exec = nullptr;
+        e = PANO_ERR_HIP;
+    }
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e) return e;
+    b->exec_slot = b->cur;
+    b->exec_luts.assign(b->renders.size(), 0);
+    for (size_t i = 0; i < b->renders.size(); i++) b->exec_luts[i] = luts && luts[i];
+    return PANO_OK;
+}
+
+int pano_batch_replay(pano_batch* b, const uint8_t* const* luts, hipStream_t stream)
+{
+    if (!b) return PANO_ERR_ARG;
+    if (!b->exec || b->dirty || b->cur != b->exec_slot) return PANO_ERR_STATE;
+    for (size_t i = 0; i < b->renders.size(); i++) {
+        const uint8_t* lt = luts ? luts[i] : nullptr;
+        if (lt && !b->exec_luts[i]) return PANO_ERR_ARG;   // captured without the LUT pass
+        if (lt) {
+            int e = b->renders[i]->upload_luts(lt, stream);
+            if (e) return e;
+        }
+    }
+    std::vector<hipEvent_t>* ev = b->timer.begin_frame();
+    if (ev) HIP_TRY(hipEventRecord((*ev)[0], stream));
+    HIP_TRY(hipGraphLaunch(b->exec, stream));
+    if (ev)
+        for (int k = 1; k <= 4; k++) HIP_TRY(hipEventRecord((*ev)[k], stream));
     pano_batch::Slot& sl = b->slots[b->cur];
     HIP_TRY(hipEventRecord(sl.ev, stream));
     sl.used = true;

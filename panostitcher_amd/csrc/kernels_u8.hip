@@ -98,7 +98,8 @@ __device__ __forceinline__ uint32_t load_u32_off(const Pl& P, int off)
 // Level-(j+1) staging region of a quad tile: element k of this thread is
 // canvas pixel (xs[k], ys[k]) (border-resolved), ok[k] whether it exists.
 constexpr int kSW = kQTW + 2, kSH = kQTH + 2;
-constexpr int kStageIt = (kSH * kSW + 255) / 256;
+constexpr int kQT = kQTW * kQTH;   // threads of a quad-tile block, one 2x2 quad each
+constexpr int kStageIt = (kSH * kSW + kQT - 1) / kQT;
 struct StageCoords {
     int xs[kStageIt], ys[kStageIt];
     bool ok[kStageIt];
@@ -108,7 +109,7 @@ __device__ __forceinline__ StageCoords stage_coords(int sx0, int sy0, int cw, in
     StageCoords S;
 #pragma unroll
     for (int k = 0; k < kStageIt; k++) {
-        const int e = threadIdx.x + k * 256;
+        const int e = threadIdx.x + k * kQT;
         const int r = e / kSW, c = e - r * kSW;
         S.ok[k] = e < kSH * kSW;
         S.xs[k] = bwrap(sx0 + c, cw);
@@ -340,7 +341,7 @@ __global__ __launch_bounds__(kDT) void k_down_u8(const Seg* __restrict__ segs, i
 
 constexpr int kBlendStage = 4;   // cameras of a blend tile staged at once
 
-__global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, int nseg, int level)
+__global__ __launch_bounds__(kQT) void k_blend_u8(const Seg* __restrict__ segs, int nseg, int level)
 {
     __shared__ uint32_t gs[kBlendStage][kSH * kSW];
     __shared__ int4 cs[kSH * kSW];
@@ -371,7 +372,7 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
                              : make_int4(0, 0, 0, 0);
 #pragma unroll
         for (int k = 0; k < kStageIt; k++) {
-            int e = threadIdx.x + k * 256;
+            int e = threadIdx.x + k * kQT;
             if (SC.ok[k]) cs[e] = cv[k];
         }
     }
@@ -395,7 +396,7 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
             for (int s = 0; s < kBlendStage; s++)
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++) {
-                    int e = threadIdx.x + k * 256;
+                    int e = threadIdx.x + k * kQT;
                     if (s < ng && SC.ok[k]) gs[s][e] = gv[s][k];
                 }
         }
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(256) void k_blend_u8(const Seg* __restrict__ segs, 
 
 constexpr int kFinStage = 4;   // owner cameras staged at once (a tile rarely has more than 2)
 
-__global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, int nseg,
+__global__ __launch_bounds__(kQT) void k_final_u8(const Seg* __restrict__ segs, int nseg,
                                                   OutTable outs)
 {
     __shared__ uint32_t gs[kFinStage][kSH * kSW];
@@ -502,7 +503,7 @@ __global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, 
                              : make_int4(0, 0, 0, 0);
 #pragma unroll
         for (int k = 0; k < kStageIt; k++) {
-            int e = threadIdx.x + k * 256;
+            int e = threadIdx.x + k * kQT;
             if (SC.ok[k]) cs[e] = cv[k];
         }
     }
@@ -532,7 +533,7 @@ __global__ __launch_bounds__(256) void k_final_u8(const Seg* __restrict__ segs, 
             for (int s = 0; s < kFinStage; s++)
 #pragma unroll
                 for (int k = 0; k < kStageIt; k++) {
-                    int e = threadIdx.x + k * 256;
+                    int e = threadIdx.x + k * kQT;
                     if (s < ng && SC.ok[k]) gs[s][e] = gv[s][k];
                 }
             __syncthreads();
@@ -590,10 +591,10 @@ hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int leve
         hipLaunchKernelGGL(k_down_u8, dim3(blocks), dim3(kDT), 0, s, segs, nseg);
         break;
     case kStageBlend:
-        hipLaunchKernelGGL(k_blend_u8, dim3(blocks), dim3(256), 0, s, segs, nseg, level);
+        hipLaunchKernelGGL(k_blend_u8, dim3(blocks), dim3(kQT), 0, s, segs, nseg, level);
         break;
     case kStageFinal:
-        hipLaunchKernelGGL(k_final_u8, dim3(blocks), dim3(256), 0, s, segs, nseg, outs);
+        hipLaunchKernelGGL(k_final_u8, dim3(blocks), dim3(kQT), 0, s, segs, nseg, outs);
         break;
     default:
         return hipErrorInvalidValue;

@@ -352,3 +352,39 @@ def test_server_results_same_alone_and_in_the_mix():
     for key, o in outs.items():
         k, f = key[0], key[1]
         np.testing.assert_array_equal(from_dev(o), rigs[k][2][f][2])
+
+
+def test_batch_graph_replays_match_render_alone():
+    # The captured per-frame graph, replayed on frames refreshed in place and
+    # with new LUT contents, gives each render's result alone; torch work and
+    # synchronisation run between the replays.
+    cases = [("fisheye2", 0.25), ("rect6", 0.25), ("mixed8", 0.125)]
+    renders, wls, maps = [], [], []
+    for name, scale in cases:
+        wl = synth.workload(name, scale)
+        xs, ys, ms = synth.maps_and_masks(wl)
+        r = P.PanoramaRender()
+        assert r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h),
+                              xs, ys, ms, wl.levels)
+        renders.append(r)
+        wls.append(wl)
+    b = P.Batch(renders)
+    bufs = [[to_dev(synth.bgrx(s)) for s in synth.frames(wl, 0)] for wl in wls]
+    outs = [torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda") for wl in wls]
+    note("batch graph capture")
+    side = torch.cuda.Stream()
+    torch.cuda.synchronize()
+    b.capture(bufs, outs, [synth.luts(wl, 0) for wl in wls], stream=side)
+    for f in range(3):
+        for k, wl in enumerate(wls):
+            for t, s in zip(bufs[k], synth.frames(wl, f)):
+                t.copy_(to_dev(synth.bgrx(s)))
+            outs[k].zero_()
+        torch.cuda.synchronize()
+        luts = [synth.luts(wl, f) for wl in wls]
+        b.replay(luts, stream=side)
+        torch.cuda.synchronize()
+        for k, (wl, r) in enumerate(zip(wls, renders)):
+            alone = torch.zeros_like(outs[k])
+            assert r.render([to_dev(synth.bgrx(s)) for s in synth.frames(wl, f)], alone, luts[k])
+            np.testing.assert_array_equal(from_dev(outs[k]), from_dev(alone))
