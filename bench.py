@@ -130,19 +130,26 @@ def timed(fn, steps, warmup):
     return time.perf_counter() - t0
 
 
-def plain_dropin(wl, xs, ys, ms, dsrc, steps, luts=None):
+def plain_dropin(wl, xs, ys, ms, dsrc, steps, luts=None, float_weights=False):
     """The reference's GPU API used as CudaPanoramaRender2 does: cudaTransform
     (when gains are given) + cudaReprojectTo16S per camera into full-canvas 16SC4
-    images, then CudaTilingMultibandBlendFast::blend (RicohUtil.cpp:1301-1313)."""
+    images, then CudaTilingMultibandBlendFast::blend (RicohUtil.cpp:1301-1313).
+    float_weights: cudaReproject to 8UC4 images and the float-weight blender
+    CudaTilingMultibandBlendFast32F instead."""
     import torch
 
     import panostitcher_amd as P
 
-    b = P.CudaTilingMultibandBlendFast()
+    if float_weights:
+        b = P.CudaTilingMultibandBlendFast32F()
+        rep_type, reproject = torch.uint8, P.cuda_reproject
+    else:
+        b = P.CudaTilingMultibandBlendFast()
+        rep_type, reproject = torch.int16, P.cuda_reproject_to16s
     assert b.prepare(ms, wl.levels)
     dx = [torch.from_numpy(x).cuda() for x in xs]
     dy = [torch.from_numpy(y).cuda() for y in ys]
-    rep = [torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.int16, device="cuda") for _ in xs]
+    rep = [torch.empty((wl.dst_h, wl.dst_w, 4), dtype=rep_type, device="cuda") for _ in xs]
     adj = [torch.empty_like(s) for s in dsrc] if luts is not None else dsrc
     out = torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
 
@@ -150,14 +157,15 @@ def plain_dropin(wl, xs, ys, ms, dsrc, steps, luts=None):
         for i in range(wl.n):
             if luts is not None:
                 P.cuda_transform(dsrc[i], luts[i], adj[i])
-            P.cuda_reproject_to16s(adj[i], dx[i], dy[i], rep[i])
+            reproject(adj[i], dx[i], dy[i], rep[i])
         b.blend(rep, out)
 
     dt = timed(frame, steps, 2)
+    path = ("transform + " if luts is not None else "")
+    path += ("reproject (8UC4) per camera + blender32f.blend, full-canvas float planes" if float_weights
+             else "reproject_to16s per camera + blender.blend, full-canvas 16-bit planes")
     return {"value": round(wl.out_pixels * steps / dt / 1e6, 2), "unit": "Mpix/s",
-            "ms_per_frame": round(dt * 1e3 / steps, 4),
-            "path": ("transform + " if luts is not None else "") +
-                    "reproject_to16s per camera + blender.blend, full-canvas 16-bit planes"}
+            "ms_per_frame": round(dt * 1e3 / steps, 4), "path": path}
 
 
 def roofline(model, per_stage_ms, frames_per_launch, own_bytes, traffic):
@@ -309,11 +317,13 @@ def run_batched(args):
     model = stage_model_bytes(wl, xs, ys, ms, r0.num_levels)
     own = [r0.model_bytes(k) for k in range(4)]
     roof = roofline(model, per_frame, S, own, read_traffic(wl.name))
-    plain = None
+    plain = plain32 = None
     if not args.no_plain:
         log("plain drop-in ...")
         plain = plain_dropin(wl, xs, ys, ms, dsrc, max(3, args.steps // 4), luts)
         log(f"plain {plain}")
+        plain32 = plain_dropin(wl, xs, ys, ms, dsrc, max(3, args.steps // 4), luts, float_weights=True)
+        log(f"plain float weights {plain32}")
     cpu = None
     if not args.no_cpu and args.cpu_frames > 0:
         log("cpu baseline ...")
@@ -335,8 +345,10 @@ def run_batched(args):
     line["batched_eager"] = eager
     line["batched_graph"] = graph
     line["plain_dropin"] = plain
+    line["plain_dropin_f32"] = plain32
     if plain:
         line["vs_plain_dropin"] = round(value / plain["value"], 3)
+        line["vs_plain_dropin_f32"] = round(value / plain32["value"], 3)
     line["roofline"] = roof
     line["cpu_baseline"] = cpu
     return line
@@ -386,6 +398,9 @@ def run_single(args, name, label, with_plain):
     line["roofline"] = roofline(model, per_frame, 1, own, read_traffic(wl.name))
     if with_plain:
         line["plain_dropin"] = plain_dropin(wl, xs, ys, ms, dsrc, max(3, args.steps // 2), luts)
+        # the same drop-in with the float-weight blender (float32 weights)
+        line["plain_dropin_f32"] = plain_dropin(wl, xs, ys, ms, dsrc, max(3, args.steps // 2), luts,
+                                                float_weights=True)
     line["cpu_baseline"] = None
     if not args.no_cpu and args.cpu_frames > 0:
         line["cpu_baseline"] = cpu_baseline(wl, srcs, xs, ys, ms, luts, args.cpu_frames)

@@ -48,6 +48,7 @@ extern "C" {
 #define PANO_8UC3 16
 #define PANO_8UC4 24
 #define PANO_16SC4 27
+#define PANO_32FC4 29
 
 const char* pano_status_string(int status);
 int pano_version(void);
@@ -105,6 +106,30 @@ int pano_blender_num_levels(const pano_blender* b);
 int64_t pano_blender_mem_bytes(const pano_blender* b);
 int64_t pano_estimate_mem_bytes(int width, int height, int num);
 
+/* ---- float-weight multiband blender: CudaTilingMultibandBlendFast32F -----
+ * Replaces the class in CudaAccel/CudaInterface.h:84-101 (implementation
+ * CudaAccel/CudaMultibandBlend.cpp:1041-1179, kernels CudaAccel/CudaPyramid6.cu),
+ * with its float arithmetic: float weight and inverse-alpha pyramids, float
+ * Laplacian pyramids, round-to-nearest-even 8-bit output. */
+typedef struct pano_blender32f pano_blender32f;
+
+int pano_blender32f_create(pano_blender32f** out);
+void pano_blender32f_destroy(pano_blender32f* b);
+
+/* prepare(const vector<Mat>& masks, int maxLevels, int minLength)
+ * (CudaInterface.h:88): host masks, num planes of rows x cols bytes.  The
+ * pyramids are built on the device; the call returns when they are done. */
+int pano_blender32f_prepare(pano_blender32f* b, int num, int rows, int cols,
+                            const uint8_t* const* masks, const size_t* mask_steps, int max_levels,
+                            int min_length);
+
+/* blend(const vector<GpuMat>& images, GpuMat& blendImage) (CudaInterface.h:89).
+ * images: device, num images of rows x cols, type PANO_8UC4 or PANO_32FC4.
+ * out: device rows x cols PANO_8UC4 (all four channels blended). */
+int pano_blender32f_blend(pano_blender32f* b, int num, const void* const* images, const size_t* steps,
+                          int type, uint8_t* out, size_t out_step, hipStream_t stream);
+int pano_blender32f_num_levels(const pano_blender32f* b);
+
 /* ---- free functions ----------------------------------------------------- */
 
 /* cudaReprojectTo16S(src, dst, xmap, ymap, stream) (CudaInterface.h:150-151),
@@ -115,6 +140,14 @@ int64_t pano_estimate_mem_bytes(int width, int height, int num);
 int pano_reproject_to16s(const uint8_t* src, size_t src_step, int src_w, int src_h,
                          const float* xmap, const float* ymap, size_t map_step, int16_t* dst,
                          size_t dst_step, int dst_w, int dst_h, hipStream_t stream);
+
+/* cudaReproject(src, dst, xmap, ymap, stream) (CudaInterface.h:147-148), with
+ * the same bilinear arithmetic as pano_reproject_to16s (the reference GPU
+ * samples bicubic, CudaReproject.cu:714-741): dst is device
+ * PANO_8UC4 dst_w x dst_h (channel 3 zero), zero where the map is invalid. */
+int pano_reproject(const uint8_t* src, size_t src_step, int src_w, int src_h, const float* xmap,
+                   const float* ymap, size_t map_step, uint8_t* dst, size_t dst_step, int dst_w, int dst_h,
+                   hipStream_t stream);
 
 /* cudaTransform(src, dst, luts, stream) (CudaInterface.h:120-121):
  * per-channel LUTs on BGRx (channel 3 copied).  luts: host, 3 x 256 bytes

@@ -25,6 +25,7 @@
  *   orc_true_num_levels        closed-form values
  *   orc_map_*                  geometric invariants of the equirect chain
  *   orc_gain_* / orc_cv_solve  see gain.c
+ *   orc_*32f                   see blend32f.c
  */
 #ifndef PANO_ORACLE_H
 #define PANO_ORACLE_H
@@ -126,6 +127,12 @@ int orc_gain_from_sums(int n, const int64_t* sums, int tint, double* gains, doub
 int orc_gain_estimate(int n, int rows, int cols, const uint8_t* const* bgr,
                       const uint8_t* const* masks, int tint, double* gains, double* rg, double* bg,
                       int* max_index, uint8_t* luts);
+
+/* Float-weight blender (blend32f.c): CudaTilingMultibandBlendFast32F. */
+void orc_pyrdown32f(const float* src, int sw, int sh, int cn, float* dst);
+void orc_pyrup32f(const float* src, int sw, int sh, int cn, float* dst, int dw, int dh);
+int orc_blend32f(int n, int rows, int cols, const uint8_t* masks, const float* images,
+                 int max_levels, int min_length, uint8_t* out);
 
 #ifdef __cplusplus
 }

@@ -76,6 +76,10 @@ def lib():
         L.orc_gain_sums.argtypes = [i, i, i, vp, vp, vp]
         L.orc_gain_from_sums.argtypes = [i, vp, i, vp, vp, vp, C.POINTER(i), vp]
         L.orc_gain_from_sums.restype = i
+        L.orc_pyrdown32f.argtypes = [vp, i, i, i, vp]
+        L.orc_pyrup32f.argtypes = [vp, i, i, i, vp, i, i]
+        L.orc_blend32f.argtypes = [i, i, i, vp, vp, i, i, vp]
+        L.orc_blend32f.restype = i
         _lib = L
     return _lib
 
@@ -298,6 +302,37 @@ def gain_from_sums(sums: np.ndarray, tint: bool = True):
 def gain_estimate(bgr, masks, tint: bool = True):
     """Gains from overlap means of reprojected BGR images (see gain.c)."""
     return gain_from_sums(gain_sums(bgr, masks), tint)
+
+
+def pyrdown32f(src: np.ndarray) -> np.ndarray:
+    """pyrDown32FC1/32FC4 of the float blender on an (h, w) or (h, w, cn) float32 image."""
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    h, w = src.shape[:2]
+    cn = 1 if src.ndim == 2 else src.shape[2]
+    out = np.zeros(((h + 1) // 2, (w + 1) // 2) + src.shape[2:], np.float32)
+    lib().orc_pyrdown32f(_p(src), w, h, cn, _p(out))
+    return out
+
+
+def pyrup32f(src: np.ndarray, dshape) -> np.ndarray:
+    src = np.ascontiguousarray(src, dtype=np.float32)
+    h, w = src.shape[:2]
+    cn = 1 if src.ndim == 2 else src.shape[2]
+    dh, dw = dshape
+    out = np.zeros((dh, dw) + src.shape[2:], np.float32)
+    lib().orc_pyrup32f(_p(src), w, h, cn, _p(out), dw, dh)
+    return out
+
+
+def blend32f(masks: np.ndarray, images: np.ndarray, max_levels: int, min_length: int = 2):
+    """CudaTilingMultibandBlendFast32F prepare + blend: masks (n, h, w) u8,
+    images (n, h, w, 4) u8 or float32 -> (h, w, 4) u8 and the level count."""
+    masks = np.ascontiguousarray(masks, dtype=np.uint8)
+    images = np.ascontiguousarray(images, dtype=np.float32)
+    n, h, w = masks.shape
+    out = np.zeros((h, w, 4), np.uint8)
+    L = lib().orc_blend32f(n, h, w, _p(masks), _p(images), max_levels, min_length, _p(out))
+    return out, L
 
 
 class PhotoParam(C.Structure):

@@ -32,6 +32,7 @@ PANO_OK = 0
 PANO_ERR_BUDGET = -6
 PANO_8UC4 = 24
 PANO_16SC4 = 27
+PANO_32FC4 = 29
 
 _lib = None
 
@@ -74,6 +75,12 @@ def lib():
             "pano_blender_unique_masks": ([vp, pp, psz], i),
             "pano_blender_num_levels": ([vp], i),
             "pano_blender_mem_bytes": ([vp], i64),
+            "pano_reproject": ([vp, sz, i, i, vp, vp, sz, vp, sz, i, i, vp], i),
+            "pano_blender32f_create": ([pp], i),
+            "pano_blender32f_destroy": ([vp], None),
+            "pano_blender32f_prepare": ([vp, i, i, i, pp, psz, i, i], i),
+            "pano_blender32f_blend": ([vp, i, pp, psz, i, vp, sz, vp], i),
+            "pano_blender32f_num_levels": ([vp], i),
             "pano_estimate_mem_bytes": ([i, i, i], i64),
             "pano_reproject_to16s": ([vp, sz, i, i, vp, vp, sz, vp, sz, i, i, vp], i),
             "pano_transform_lut3": ([vp, sz, vp, sz, i, i, vp, vp], i),
@@ -140,6 +147,8 @@ def exported_symbols():
         "pano_batch_capture", "pano_batch_replay",
         "pano_server_create", "pano_server_destroy", "pano_server_admit", "pano_server_retire",
         "pano_server_submit", "pano_server_poll", "pano_server_drain", "pano_server_stats",
+        "pano_blender32f_create", "pano_blender32f_destroy", "pano_blender32f_prepare",
+        "pano_blender32f_blend", "pano_blender32f_num_levels", "pano_reproject",
     ]
 
 
@@ -315,6 +324,61 @@ class CudaTilingMultibandBlendFast:
         return lib().pano_estimate_mem_bytes(width, height, num_images)
 
 
+class CudaTilingMultibandBlendFast32F:
+    """CudaTilingMultibandBlendFast32F (CudaAccel/CudaInterface.h:84-101): the
+    float-weight blender; blend() takes CV_8UC4 or CV_32FC4 images."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().pano_blender32f_create(C.byref(h)), "create blender32f")
+        self._h = h
+        self.num_images = 0
+        self.rows = self.cols = 0
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pano_blender32f_destroy(h)
+            self._h = None
+
+    def prepare(self, masks, max_levels: int, min_length: int = 2) -> bool:
+        """masks: list of HxW uint8 host arrays."""
+        if len(masks) == 0:
+            return False
+        rows, cols = masks[0].shape[:2]
+        arrs = [np.ascontiguousarray(m, np.uint8) for m in masks]
+        if any(a.shape != (rows, cols) for a in arrs):
+            return False
+        st = lib().pano_blender32f_prepare(self._h, len(arrs), rows, cols,
+                                           _ptrs([a.ctypes.data for a in arrs]),
+                                           _sizes([a.strides[0] for a in arrs]), max_levels,
+                                           min_length)
+        if st == PANO_OK:
+            self.num_images, self.rows, self.cols = len(masks), rows, cols
+        return st == PANO_OK
+
+    @property
+    def num_levels(self):
+        return lib().pano_blender32f_num_levels(self._h)
+
+    def blend(self, images, out, stream=None) -> bool:
+        """images: list of HxWx4 uint8 or float32 cuda tensors; out: HxWx4 uint8."""
+        import torch
+
+        t = images[0]
+        if t.shape[-1] != 4 or t.dtype not in (torch.uint8, torch.float32):
+            raise ValueError("images must be HxWx4 uint8 (CV_8UC4) or float32 (CV_32FC4)")
+        typ = PANO_8UC4 if t.dtype == torch.uint8 else PANO_32FC4
+        ptrs, steps = zip(*[_dev_ptr_step(x) for x in images])
+        op, ostep = _dev_ptr_step(out)
+        st = lib().pano_blender32f_blend(self._h, len(images), _ptrs(ptrs), _sizes(steps), typ,
+                                         C.c_void_p(op), ostep, _stream(stream))
+        if st == -2:
+            return False
+        _check(st, "blend")
+        return True
+
+
 # ------------------------------------------------------------- free functions --
 
 def cuda_reproject_to16s(src, xmap, ymap, dst=None, stream=None):
@@ -332,6 +396,24 @@ def cuda_reproject_to16s(src, xmap, ymap, dst=None, stream=None):
                                       C.c_void_p(ymap.data_ptr()), xmap.stride(0) * 4,
                                       C.c_void_p(dp), ds, dw, dh, _stream(stream)),
            "cudaReprojectTo16S")
+    return dst
+
+
+def cuda_reproject(src, xmap, ymap, dst=None, stream=None):
+    """cudaReproject(src 8UC4, dst 8UC4, xmap, ymap) with bilinear sampling."""
+    import torch
+
+    sh, sw = src.shape[:2]
+    dh, dw = xmap.shape
+    if dst is None:
+        dst = torch.empty((dh, dw, 4), dtype=torch.uint8, device=src.device)
+    sp, ss = _dev_ptr_step(src)
+    dp, ds = _dev_ptr_step(dst)
+    assert xmap.stride(0) == ymap.stride(0)
+    _check(lib().pano_reproject(C.c_void_p(sp), ss, sw, sh, C.c_void_p(xmap.data_ptr()),
+                                C.c_void_p(ymap.data_ptr()), xmap.stride(0) * 4, C.c_void_p(dp), ds,
+                                dw, dh, _stream(stream)),
+           "cudaReproject")
     return dst
 
 

@@ -1000,6 +1000,20 @@ int pano_reproject_to16s(const uint8_t* src, size_t src_step, int src_w, int src
     return PANO_OK;
 }
 
+int pano_reproject(const uint8_t* src, size_t src_step, int src_w, int src_h, const float* xmap,
+                   const float* ymap, size_t map_step, uint8_t* dst, size_t dst_step, int dst_w, int dst_h,
+                   hipStream_t stream)
+{
+    if (!src || !xmap || !ymap || !dst || src_w <= 0 || src_h <= 0 || dst_w <= 0 || dst_h <= 0)
+        return PANO_ERR_ARG;
+    if (src_step < (size_t)src_w * 4 || src_step % 4 || map_step < (size_t)dst_w * 4 ||
+        map_step % 4 || dst_step < (size_t)dst_w * 4 || dst_step % 4 || src_step > 0xffffffffu)
+        return PANO_ERR_ARG;
+    HIP_TRY(launch_reproject8u(src, src_step, src_w, src_h, xmap, ymap, map_step, dst, dst_step, dst_w,
+                               dst_h, stream));
+    return PANO_OK;
+}
+
 int pano_transform_lut3(const uint8_t* src, size_t src_step, uint8_t* dst, size_t dst_step, int w,
                         int h, const uint8_t* luts, hipStream_t stream)
 {

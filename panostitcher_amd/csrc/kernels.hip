@@ -50,24 +50,26 @@ __global__ __launch_bounds__(256) void k_remap(const RemapJob* __restrict__ jobs
 }
 
 // cudaReprojectTo16S replacement: full-canvas float32 maps -> 16SC4 (x = 0).
-__global__ __launch_bounds__(256) void k_reproject16s(const uint8_t* __restrict__ src,
-                                                      uint32_t src_step, int sw, int sh,
-                                                      const float* __restrict__ xmap,
-                                                      const float* __restrict__ ymap,
-                                                      size_t map_step, int16_t* __restrict__ dst,
-                                                      size_t dst_step, int dw, int dh)
+// One output pixel per thread: 16-bit BGRx (short4) or 8-bit BGRx (uint32),
+// channel 3 zero, all zero where the map is invalid.
+template <bool U8>
+__global__ __launch_bounds__(256) void k_reproject(const uint8_t* __restrict__ src, uint32_t src_step,
+                                                   int sw, int sh, const float* __restrict__ xmap,
+                                                   const float* __restrict__ ymap, size_t map_step,
+                                                   void* __restrict__ dst, size_t dst_step, int dw, int dh)
 {
     int X = blockIdx.x * 64 + (threadIdx.x & 63);
     int Y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (X >= dw || Y >= dh) return;
     double x = ((const float*)((const uint8_t*)xmap + (size_t)Y * map_step))[X];
     double y = ((const float*)((const uint8_t*)ymap + (size_t)Y * map_step))[X];
-    short4 o = make_short4(0, 0, 0, 0);
-    if (x >= 0 && y >= 0 && x < sw && y < sh) {
-        uint32_t v = bilinear_bgr(src, src_step, sw, sh, x, y, nullptr);
-        o = make_short4((short)(v & 255), (short)((v >> 8) & 255), (short)((v >> 16) & 255), 0);
-    }
-    ((short4*)((uint8_t*)dst + (size_t)Y * dst_step))[X] = o;
+    uint32_t v = 0;
+    if (x >= 0 && y >= 0 && x < sw && y < sh) v = bilinear_bgr(src, src_step, sw, sh, x, y, nullptr) & 0xffffffu;
+    uint8_t* row = (uint8_t*)dst + (size_t)Y * dst_step;
+    if (U8)
+        ((uint32_t*)row)[X] = v;
+    else
+        ((short4*)row)[X] = make_short4((short)(v & 255), (short)((v >> 8) & 255), (short)((v >> 16) & 255), 0);
 }
 
 // cudaTransform replacement: per-channel LUT on BGRx; channel 3 copied.
@@ -421,8 +423,18 @@ hipError_t launch_reproject16s(const uint8_t* src, size_t src_step, int sw, int 
                                int16_t* dst, size_t dst_step, int dw, int dh, hipStream_t s)
 {
     dim3 g((dw + 63) / 64, (dh + 3) / 4);
-    hipLaunchKernelGGL(k_reproject16s, g, dim3(256), 0, s, src, (uint32_t)src_step, sw, sh, xmap,
-                       ymap, map_step, dst, dst_step, dw, dh);
+    hipLaunchKernelGGL(k_reproject<false>, g, dim3(256), 0, s, src, (uint32_t)src_step, sw, sh, xmap,
+                       ymap, map_step, (void*)dst, dst_step, dw, dh);
+    return hipGetLastError();
+}
+
+hipError_t launch_reproject8u(const uint8_t* src, size_t src_step, int sw, int sh, const float* xmap,
+                              const float* ymap, size_t map_step, uint8_t* dst, size_t dst_step, int dw,
+                              int dh, hipStream_t s)
+{
+    dim3 g((dw + 63) / 64, (dh + 3) / 4);
+    hipLaunchKernelGGL(k_reproject<true>, g, dim3(256), 0, s, src, (uint32_t)src_step, sw, sh, xmap,
+                       ymap, map_step, (void*)dst, dst_step, dw, dh);
     return hipGetLastError();
 }
 

@@ -29,6 +29,10 @@ hipError_t launch_reproject16s(const uint8_t* src, size_t src_step, int sw, int 
                                const float* xmap, const float* ymap, size_t map_step,
                                int16_t* dst, size_t dst_step, int dw, int dh, hipStream_t s);
 
+hipError_t launch_reproject8u(const uint8_t* src, size_t src_step, int sw, int sh, const float* xmap,
+                              const float* ymap, size_t map_step, uint8_t* dst, size_t dst_step, int dw,
+                              int dh, hipStream_t s);
+
 hipError_t launch_transform(const uint8_t* src, size_t src_step, uint8_t* dst, size_t dst_step,
                             int w, int h, const uint8_t* lut_dev, hipStream_t s);
 
@@ -51,6 +55,25 @@ hipError_t launch_gain(const Seg* segs, int nseg, int blocks, const RenderDesc* 
 
 // Zero-fill with a kernel (stays correct inside captured graphs).
 hipError_t launch_zero(void* p, size_t bytes, hipStream_t s);
+
+// Float-weight blender (kernels_f32.hip).  Planes are pitched float4 images
+// (float for weights, alphas and sums); the 8-bit level-0 image is read as
+// BGRx bytes when u8 is set.
+hipError_t launch_f32_down(int channels, bool u8, const void* src, int sw, int sh, size_t sstep,
+                           void* dst, int dw, int dh, size_t dstep, const float* scale, size_t scstep,
+                           hipStream_t s);
+hipError_t launch_f32_lap(bool u8, const void* g, size_t gstep, const void* g1, size_t g1step, int w1,
+                          int h1, const float* wt, size_t wstep, void* r, size_t rstep, int w, int h,
+                          hipStream_t s);
+hipError_t launch_f32_collapse(const void* c1, size_t c1step, int w1, int h1, const float* winv,
+                               size_t wistep, void* r, size_t rstep, int w, int h, hipStream_t s);
+hipError_t launch_f32_out(const void* c0, size_t cstep, const uint8_t* cover, int cpitch, uint8_t* out,
+                          size_t ostep, int w, int h, hipStream_t s);
+hipError_t launch_f32_from_mask(const uint8_t* m, int mpitch, float* f, size_t fstep, int w, int h,
+                                hipStream_t s);
+hipError_t launch_f32_alpha(float* a, size_t astep, float* bin, size_t bstep, int w, int h, hipStream_t s);
+hipError_t launch_f32_wsum(const float* wsrc, size_t wstep, float* sum, size_t sstep, int w, int h,
+                           bool invert, hipStream_t s);
 
 hipError_t launch_copy_plane(const void* src, size_t src_step, void* dst, size_t dst_step,
                              int row_bytes, int rows, hipStream_t s);

@@ -53,6 +53,11 @@ def test_reproject_to16s_matches_oracle():
     got = from_dev(out)
     np.testing.assert_array_equal(got[..., :3], ref)
     assert not got[..., 3].any()
+    # cudaReproject: the same samples as bytes
+    note("reproject 8UC4 53x37 -> 40x24")
+    got8 = from_dev(P.cuda_reproject(to_dev(synth.bgrx(src)), to_dev(xm), to_dev(ym)))
+    np.testing.assert_array_equal(got8[..., :3], ref)
+    assert not got8[..., 3].any()
 
 
 def test_transform_matches_oracle():
