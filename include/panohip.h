@@ -91,6 +91,17 @@ int pano_blender_prepare_device(pano_blender* b, int num, int rows, int cols,
 int pano_blender_blend(pano_blender* b, int num, const void* const* images, const size_t* steps,
                        int type, uint8_t* out, size_t out_step, hipStream_t stream);
 
+/* blend(const vector<GpuMat>& images, const vector<GpuMat>& masks, GpuMat&
+ * blendImage) (CudaInterface.h:55-56, CudaMultibandBlend.cpp:825-893): the
+ * weights come from this call's masks (device, num planes of rows x cols
+ * bytes), always divided by their sums, with the integer arithmetic of
+ * Blend/MultiBandBlend.cpp:1033-1102; the image pyramids keep the prepared
+ * alphas.  The output is zero where none of these masks is set.  Device
+ * buffers for the weights are allocated on the first call and kept. */
+int pano_blender_blend_custom(pano_blender* b, int num, const void* const* images, const size_t* steps,
+                              int type, const uint8_t* const* masks, const size_t* mask_steps, uint8_t* out,
+                              size_t out_step, hipStream_t stream);
+
 /* begin() / tile(image, index) / end(blendImage) (CudaInterface.h:51-53).
  * tile() consumes the image before it returns on the stream: the caller may
  * reuse the buffer for the next camera once the stream reaches that point. */

@@ -420,11 +420,13 @@ static void orc_build_laplace(const orc_blender* b, int i, const int16_t* image,
     free(up);
 }
 
-/* blend (MultiBandBlend.cpp:987-1031). */
-int orc_blender_blend(orc_blender* b, const int16_t* images, uint8_t* out, int nthreads)
+/* blend (MultiBandBlend.cpp:987-1031) with the weight pyramids `weight`
+ * (n * (L + 1) planes); rweight NULL normalises by >> 8 (fullMask), else by the
+ * weight sums; pixels with covered[p] == 0 are zeroed unless full. */
+static int blend_with(orc_blender* b, const int16_t* images, int16_t* const* weight,
+                      int32_t* const* rweight, const uint8_t* covered, int full, uint8_t* out,
+                      int nthreads)
 {
-    if (!b || !b->n)
-        return 0;
     int L = b->levels, n = b->n;
     size_t P = (size_t)b->rows * b->cols;
     int32_t** R = (int32_t**)calloc((size_t)L + 1, sizeof(int32_t*));
@@ -449,7 +451,7 @@ int orc_blender_blend(orc_blender* b, const int16_t* images, uint8_t* out, int n
             int i = i0 + t;
             for (int j = 0; j <= L; j++) {
                 size_t np = (size_t)b->lw[j] * b->lh[j];
-                const int16_t* w = b->weight[(size_t)i * (L + 1) + j];
+                const int16_t* w = weight[(size_t)i * (L + 1) + j];
                 const int16_t* l = pyrs[t][j];
                 int32_t* r = R[j];
                 for (size_t p = 0; p < np; p++) {
@@ -470,10 +472,10 @@ int orc_blender_blend(orc_blender* b, const int16_t* images, uint8_t* out, int n
     for (int j = 0; j <= L; j++) {
         size_t np = (size_t)b->lw[j] * b->lh[j];
         int32_t* r = R[j];
-        if (b->full) {
+        if (!rweight) {
             for (size_t e = 0; e < np * 3; e++) r[e] = r[e] >> 8;
         } else {
-            const int32_t* w = b->rweight[j];
+            const int32_t* w = rweight[j];
             for (size_t p = 0; p < np; p++) {
                 int wv = w[p];
                 if (!wv) wv++;
@@ -495,10 +497,65 @@ int orc_blender_blend(orc_blender* b, const int16_t* images, uint8_t* out, int n
     /* convertTo(CV_8U) and setTo(0, maskNot) (:1028-1030) */
     for (size_t p = 0; p < P; p++)
         for (int c = 0; c < 3; c++)
-            out[p * 3 + c] = (!b->full && !b->covered[p]) ? 0 : sat_u8(R[0][p * 3 + c]);
+            out[p * 3 + c] = (!full && !covered[p]) ? 0 : sat_u8(R[0][p * 3 + c]);
     for (int j = 0; j <= L; j++) free(R[j]);
     free(R);
     return 1;
+}
+
+int orc_blender_blend(orc_blender* b, const int16_t* images, uint8_t* out, int nthreads)
+{
+    if (!b || !b->n)
+        return 0;
+    return blend_with(b, images, b->weight, b->full ? NULL : b->rweight, b->covered, b->full, out,
+                      nthreads);
+}
+
+/* blend(images, masks, blendImage) (MultiBandBlend.cpp:1033-1102): weights
+ * from the call's masks (Gaussian pyramids of 256 * mask, createGaussPyramid
+ * :64-74), always normalised by their sums (normalize(pyr, weight) :203-241),
+ * alphas from prepare; zero where no call mask covers the pixel unless they
+ * cover the canvas. */
+int orc_blender_blend_custom(orc_blender* b, const int16_t* images, const uint8_t* masks,
+                             uint8_t* out, int nthreads)
+{
+    if (!b || !b->n)
+        return 0;
+    const int L = b->levels, n = b->n;
+    const size_t P = (size_t)b->rows * b->cols;
+    int16_t** w = (int16_t**)calloc((size_t)n * (L + 1), sizeof(int16_t*));
+    int32_t** rw = (int32_t**)calloc((size_t)L + 1, sizeof(int32_t*));
+    uint8_t* cov = (uint8_t*)calloc(P, 1);
+    size_t cnt = 0;
+    for (size_t p = 0; p < P; p++) {
+        uint8_t v = 0;
+        for (int i = 0; i < n; i++) v |= masks[P * i + p];
+        cov[p] = v;
+        cnt += v != 0;
+    }
+    for (int i = 0; i < n; i++) {
+        int16_t** wp = w + (size_t)i * (L + 1);
+        wp[0] = (int16_t*)malloc(sizeof(int16_t) * P);
+        for (size_t p = 0; p < P; p++) wp[0][p] = masks[P * i + p] ? 256 : 0;
+        for (int j = 0; j < L; j++) {
+            wp[j + 1] = (int16_t*)malloc(sizeof(int16_t) * (size_t)b->lw[j + 1] * b->lh[j + 1]);
+            orc_pyrdown_s16(wp[j], b->lw[j], b->lh[j], b->lw[j], 1, wp[j + 1], b->lw[j + 1],
+                            b->lh[j + 1], b->lw[j + 1], ORC_BORDER_WRAP, ORC_BORDER_REFLECT_101);
+        }
+    }
+    for (int j = 0; j <= L; j++) {
+        size_t np = (size_t)b->lw[j] * b->lh[j];
+        rw[j] = (int32_t*)calloc(np, sizeof(int32_t));
+        for (int i = 0; i < n; i++)
+            for (size_t p = 0; p < np; p++) rw[j][p] += w[(size_t)i * (L + 1) + j][p];
+    }
+    int ok = blend_with(b, images, w, rw, cov, cnt == P, out, nthreads);
+    for (size_t k = 0; k < (size_t)n * (L + 1); k++) free(w[k]);
+    for (int j = 0; j <= L; j++) free(rw[j]);
+    free(w);
+    free(rw);
+    free(cov);
+    return ok;
 }
 
 int orc_blender_num_levels(const orc_blender* b) { return b->levels; }

@@ -70,6 +70,8 @@ int orc_blender_prepare(orc_blender* b, int n, int rows, int cols, const uint8_t
  * nthreads > 1 builds the per-camera pyramids concurrently, the way
  * TilingMultibandBlendFastParallel does; the arithmetic is identical. */
 int orc_blender_blend(orc_blender* b, const int16_t* images, uint8_t* out, int nthreads);
+int orc_blender_blend_custom(orc_blender* b, const int16_t* images, const uint8_t* masks,
+                             uint8_t* out, int nthreads);
 int orc_blender_num_levels(const orc_blender* b);
 int orc_blender_full_mask(const orc_blender* b);
 void orc_blender_unique_masks(const orc_blender* b, uint8_t* out);

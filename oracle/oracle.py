@@ -49,6 +49,8 @@ def lib():
         L.orc_blender_prepare.restype = i
         L.orc_blender_blend.argtypes = [vp, vp, vp, i]
         L.orc_blender_blend.restype = i
+        L.orc_blender_blend_custom.argtypes = [vp, vp, vp, vp, i]
+        L.orc_blender_blend_custom.restype = i
         L.orc_blender_num_levels.argtypes = [vp]
         L.orc_blender_num_levels.restype = i
         L.orc_blender_full_mask.argtypes = [vp]
@@ -197,6 +199,16 @@ class Blender:
         assert images.shape == (self.n, self.rows, self.cols, 3)
         out = np.zeros((self.rows, self.cols, 3), np.uint8)
         assert lib().orc_blender_blend(self.h, _p(images), _p(out), nthreads)
+        return out
+
+    def blend_custom(self, images: np.ndarray, masks: np.ndarray, nthreads: int = 1) -> np.ndarray:
+        """blend(images, masks, blendImage): weights from these masks."""
+        images = np.ascontiguousarray(images, dtype=np.int16)
+        masks = np.ascontiguousarray(masks, dtype=np.uint8)
+        assert images.shape == (self.n, self.rows, self.cols, 3)
+        assert masks.shape == (self.n, self.rows, self.cols)
+        out = np.zeros((self.rows, self.cols, 3), np.uint8)
+        assert lib().orc_blender_blend_custom(self.h, _p(images), _p(masks), _p(out), nthreads)
         return out
 
     def render(self, srcs, xmaps, ymaps, luts=None, nthreads=1) -> np.ndarray:

@@ -69,6 +69,7 @@ def lib():
             "pano_blender_prepare": ([vp, i, i, i, pp, psz, i, i], i),
             "pano_blender_prepare_device": ([vp, i, i, i, pp, psz, i, i], i),
             "pano_blender_blend": ([vp, i, pp, psz, i, vp, sz, vp], i),
+            "pano_blender_blend_custom": ([vp, i, pp, psz, i, pp, psz, vp, sz, vp], i),
             "pano_blender_begin": ([vp], i),
             "pano_blender_tile": ([vp, vp, sz, i, i, vp], i),
             "pano_blender_end": ([vp, vp, sz, vp], i),
@@ -149,6 +150,7 @@ def exported_symbols():
         "pano_server_submit", "pano_server_poll", "pano_server_drain", "pano_server_stats",
         "pano_blender32f_create", "pano_blender32f_destroy", "pano_blender32f_prepare",
         "pano_blender32f_blend", "pano_blender32f_num_levels", "pano_reproject",
+        "pano_blender_blend_custom",
     ]
 
 
@@ -294,6 +296,22 @@ class CudaTilingMultibandBlendFast:
         if st == -2:
             return False
         _check(st, "blend")
+        return True
+
+    def blend_with_masks(self, images, masks, out, stream=None) -> bool:
+        """blend(images, masks, blendImage) (CudaInterface.h:55-56): weights from
+        these masks (HxW uint8 cuda tensors) instead of the prepared seams."""
+        typ, _ = self._type(images[0])
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in images])
+        mptrs = [m.data_ptr() for m in masks]
+        msteps = [m.stride(0) for m in masks]
+        op, ostep = _dev_ptr_step(out)
+        st = lib().pano_blender_blend_custom(self._h, len(images), _ptrs(ptrs), _sizes(steps), typ,
+                                             _ptrs(mptrs), _sizes(msteps), C.c_void_p(op), ostep,
+                                             _stream(stream))
+        if st == -2:
+            return False
+        _check(st, "blend with masks")
         return True
 
     def begin(self):

@@ -654,11 +654,47 @@ struct DeviceRig {
 
 // ------------------------------------------------------------------ objects --
 
+namespace {
+// Device buffers of blend(images, masks): per level the cameras' weight planes
+// (one after another, stride[j] elements apart) and the weight sums, all with
+// the level-0 row pitch, and the masks' union.  Allocated on first use.
+struct CustomBufs {
+    int pitch = 0;
+    std::vector<int16_t*> w;
+    std::vector<int32_t*> wsum;
+    std::vector<size_t> stride;
+    uint8_t* cover = nullptr;
+    ~CustomBufs()
+    {
+        for (int16_t* p : w) (void)hipFree(p);
+        for (int32_t* p : wsum) (void)hipFree(p);
+        if (cover) (void)hipFree(cover);
+    }
+    int alloc(const Prepared& pr)
+    {
+        const int L = pr.levels;
+        pitch = (pr.cols + 63) / 64 * 64;
+        w.assign(L + 1, nullptr);
+        wsum.assign(L + 1, nullptr);
+        stride.assign(L + 1, 0);
+        for (int j = 0; j <= L; j++) {
+            stride[j] = (size_t)pitch * pr.lh[j];
+            if (hipMalloc((void**)&w[j], stride[j] * pr.n * sizeof(int16_t)) != hipSuccess ||
+                hipMalloc((void**)&wsum[j], stride[j] * sizeof(int32_t)) != hipSuccess)
+                return PANO_ERR_NOMEM;
+        }
+        if (hipMalloc((void**)&cover, stride[0]) != hipSuccess) return PANO_ERR_NOMEM;
+        return PANO_OK;
+    }
+};
+}  // namespace
+
 struct pano_blender {
     std::unique_ptr<DeviceRig> rig;
     bool ok = false;
     bool tiling = false;
     std::vector<uint8_t> tiled;
+    std::unique_ptr<CustomBufs> custom;
 };
 
 // Events between the stages of each timed frame.
@@ -903,6 +939,50 @@ int pano_blender_blend(pano_blender* b, int num, const void* const* images, cons
     ot.p[0] = out;
     ot.step[0] = (uint32_t)out_step;
     return r.run_pyramid(type == PANO_16SC4 ? kPixS16 : kPixU8, st, ot, stream);
+}
+
+int pano_blender_blend_custom(pano_blender* b, int num, const void* const* images, const size_t* steps,
+                              int type, const uint8_t* const* masks, const size_t* mask_steps, uint8_t* out,
+                              size_t out_step, hipStream_t stream)
+{
+    if (!b) return PANO_ERR_ARG;
+    if (!b->ok) return PANO_ERR_STATE;
+    DeviceRig& r = *b->rig;
+    const Prepared& pr = r.prep;
+    if (num != pr.n || !images || !steps || !masks || !mask_steps || !out) return PANO_ERR_ARG;
+    if (type != PANO_16SC4 && type != PANO_8UC4) return PANO_ERR_ARG;
+    if (num > kMaxSrc) return PANO_ERR_UNSUPPORTED;
+    const size_t px = type == PANO_16SC4 ? 8 : 4;
+    if (out_step < (size_t)pr.cols * 4 || out_step % 4 || out_step > 0xffffffffu) return PANO_ERR_ARG;
+    SrcTable st{}, mt{};
+    for (int i = 0; i < num; i++) {
+        if (!images[i] || steps[i] < (size_t)pr.cols * px || steps[i] % px || steps[i] > 0xffffffffu)
+            return PANO_ERR_ARG;
+        if (!masks[i] || mask_steps[i] < (size_t)pr.cols || mask_steps[i] > 0xffffffffu) return PANO_ERR_ARG;
+        st.p[i] = (const uint8_t*)images[i];
+        st.step[i] = (uint32_t)steps[i];
+        mt.p[i] = masks[i];
+        mt.step[i] = (uint32_t)mask_steps[i];
+    }
+    if (!b->custom) {
+        std::unique_ptr<CustomBufs> c(new (std::nothrow) CustomBufs());
+        if (!c) return PANO_ERR_NOMEM;
+        int e = c->alloc(pr);
+        if (e) return e;
+        b->custom = std::move(c);
+    }
+    CustomBufs& cb = *b->custom;
+    const int L = pr.levels, user = type == PANO_16SC4 ? kPixS16 : kPixU8;
+    // the image pyramids with the prepared alphas
+    for (int j = 0; j < L; j++)
+        HIP_TRY(launch_down(r.kind, j == 0 ? user : -1, r.d_down[j], r.n_down[j], r.d_planes_user, st, stream));
+    HIP_TRY(launch_custom_weights(mt, num, L, pr.lw.data(), pr.lh.data(), cb.w.data(), cb.stride.data(),
+                                  cb.pitch, cb.wsum.data(), cb.cover, stream));
+    for (int j = L; j >= 0; j--)
+        HIP_TRY(launch_blend_custom(user, r.d_planes_user, r.d_canvas, num, L, j, cb.w[j], cb.stride[j],
+                                    cb.pitch, cb.wsum[j], st, cb.cover, out, out_step, pr.lw[j], pr.lh[j],
+                                    stream));
+    return PANO_OK;
 }
 
 int pano_blender_begin(pano_blender* b)

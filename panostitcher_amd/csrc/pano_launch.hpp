@@ -75,6 +75,16 @@ hipError_t launch_f32_alpha(float* a, size_t astep, float* bin, size_t bstep, in
 hipError_t launch_f32_wsum(const float* wsrc, size_t wstep, float* sum, size_t sstep, int w, int h,
                            bool invert, hipStream_t s);
 
+// Custom-mask blend of the 16-bit blender (kernels_custom.hip).  Weights,
+// sums and cover share the row pitch `pitch` (elements) at every level.
+hipError_t launch_custom_weights(const SrcTable& masks, int n, int L, const int* lw, const int* lh,
+                                 int16_t* const* wts, const size_t* cam_stride, int pitch,
+                                 int32_t* const* wsum, uint8_t* cover, hipStream_t s);
+hipError_t launch_blend_custom(int user_kind, const Plane* planes, const CanvasLevel* canvas, int n, int L,
+                               int level, const int16_t* wts, size_t cam_stride, int pitch,
+                               const int32_t* wsum, const SrcTable& src, const uint8_t* cover,
+                               uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
+
 hipError_t launch_copy_plane(const void* src, size_t src_step, void* dst, size_t dst_step,
                              int row_bytes, int rows, hipStream_t s);
 

@@ -86,11 +86,14 @@ def tdiv(a, b):
     return np.where((a < 0) != (b < 0), -q, q)
 
 
-def multiband(images, masks, max_levels, min_len=2):
-    """images: (n, H, W, 3) int16; masks (n, H, W) u8.  Returns (H, W, 3) u8."""
+def multiband(images, masks, max_levels, min_len=2, weight_masks=None):
+    """images: (n, H, W, 3) int16; masks (n, H, W) u8.  Returns (H, W, 3) u8.
+    weight_masks: the custom-mask blend (MultiBandBlend.cpp:1033-1102): weights
+    from these masks instead of the seams, always divided by their sums, and
+    the output zeroed outside their union."""
     n, H, W = masks.shape
     L = num_levels(W, H, max_levels, min_len)
-    uniq = unique_masks(masks)
+    uniq = unique_masks(masks) if weight_masks is None else weight_masks
     sizes = [(H, W)]
     for _ in range(L):
         sizes.append(((sizes[-1][0] + 1) // 2, (sizes[-1][1] + 1) // 2))
@@ -107,7 +110,7 @@ def multiband(images, masks, max_levels, min_len=2):
             ap.append(a)
             cur = np.where(a != 0, 256, 0)
         alphas.append(ap)
-    cover = np.any(masks != 0, axis=0)
+    cover = np.any((masks if weight_masks is None else weight_masks) != 0, axis=0)
     full = bool(cover.all())
     R = [np.zeros(s + (3,), np.int64) for s in sizes]
     for i in range(n):
@@ -125,7 +128,7 @@ def multiband(images, masks, max_levels, min_len=2):
         for j in range(L + 1):
             R[j] += Lp[j] * weights[i][j].astype(np.int64)[..., None]
     for j in range(L + 1):
-        if full:
+        if full and weight_masks is None:
             R[j] = R[j] >> 8
         else:
             wsum = sum(weights[i][j].astype(np.int64) for i in range(n))

@@ -112,6 +112,40 @@ def test_blender_16s_matches_oracle(seed, n, h, w, maxl, junk):
     assert_parity(from_dev(out2)[..., :3], ref, exact=True, label="tile/end")
 
 
+@pytest.mark.parametrize("seed,n,h,w,maxl,full,kind", [(0, 3, 48, 96, 5, False, "s16"),
+                                                       (1, 2, 45, 90, 4, True, "u8"),
+                                                       (2, 4, 64, 128, 16, False, "u8"),
+                                                       (3, 3, 33, 70, 3, True, "s16")])
+def test_blender_custom_masks_matches_oracle(seed, n, h, w, maxl, full, kind):
+    rng = np.random.default_rng(seed + 100)
+    masks = random_masks(rng, n, h, w)
+    custom = random_masks(rng, n, h, w)
+    if full:
+        custom[n - 1][:, :] = 255
+    imgs = rng.integers(0, 256, size=(n, h, w, 3)).astype(np.int16)
+    ob = O.Blender()
+    assert ob.prepare(masks, maxl)
+    ref = ob.blend_custom(imgs, custom)
+    note(f"blender custom masks seed={seed} n={n} {w}x{h} L={ob.num_levels} {kind}")
+    b = P.CudaTilingMultibandBlendFast()
+    assert b.prepare(list(masks), maxl)
+    if kind == "u8":
+        dev = [to_dev(synth.bgrx(imgs[i].astype(np.uint8))) for i in range(n)]
+    else:
+        dev = [to_dev(np.concatenate([imgs[i], np.full((h, w, 1), 5, np.int16)], axis=-1))
+               for i in range(n)]
+    dmasks = [to_dev(custom[i]) for i in range(n)]
+    out = torch.full((h, w, 4), 7, dtype=torch.uint8, device="cuda")
+    assert b.blend_with_masks(dev, dmasks, out)
+    got = from_dev(out)
+    assert_parity(got[..., :3], ref, exact=True, label="blend custom masks")
+    assert not got[..., 3].any()
+    # the prepared blend still gives its own result afterwards
+    out2 = torch.zeros_like(out)
+    assert b.blend(dev, out2)
+    assert_parity(from_dev(out2)[..., :3], ob.blend(imgs), exact=True, label="blend after custom")
+
+
 def test_blender_8u_input_matches_oracle():
     rng = np.random.default_rng(7)
     n, h, w = 3, 40, 80

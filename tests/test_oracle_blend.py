@@ -116,6 +116,34 @@ def test_single_full_camera_reproduces_input(h, w):
     np.testing.assert_array_equal(b.blend(img), img[0].astype(np.uint8))
 
 
+@pytest.mark.parametrize("seed,n,h,w,maxl,full", [(0, 2, 32, 64, 5, False), (1, 3, 24, 48, 3, True),
+                                                  (2, 3, 27, 45, 4, False), (3, 4, 33, 66, 9, True)])
+def test_blend_custom_masks_matches_numpy_restatement(seed, n, h, w, maxl, full):
+    rng = np.random.default_rng(seed + 40)
+    masks, imgs = random_case(rng, n, h, w, junk_outside=(seed % 2 == 1))
+    custom, _ = random_case(rng, n, h, w)
+    if full:
+        custom[0][:, :] = 255
+    b = O.Blender()
+    assert b.prepare(masks, maxl)
+    got = b.blend_custom(imgs, custom)
+    np.testing.assert_array_equal(got, np_ref.multiband(imgs, masks, maxl, weight_masks=custom))
+    if not full:
+        assert not got[~custom.any(0)].any()
+
+
+def test_blend_custom_with_the_seams_is_the_blend():
+    # When the prepare masks leave the canvas partly uncovered, the prepared
+    # blend divides by the seam weights' sums too: with the seams as custom
+    # masks the two blends are the same computation.
+    rng = np.random.default_rng(9)
+    masks, imgs = random_case(rng, 3, 40, 80)
+    masks[:, :3, :] = 0
+    b = O.Blender(); b.prepare(masks, 5)
+    assert not b.full_mask
+    np.testing.assert_array_equal(b.blend_custom(imgs, b.unique_masks()), b.blend(imgs))
+
+
 def test_blend_zero_outside_coverage_and_permutation():
     rng = np.random.default_rng(5)
     masks, imgs = random_case(rng, 3, 32, 64)
