@@ -272,6 +272,14 @@ def run_batched(args):
 
     batch = P.Batch(renders)
     lut_list = [luts] * S
+    tune = None
+    if not args.no_autotune:
+        chosen, tms = batch.autotune(stream_src, outs, lut_list, stream, iters=3)
+        tune = {"chosen": dict(zip(STAGES, chosen)),
+                "ms_per_step": {f"order{m}": dict(zip(STAGES, [round(v, 4) for v in tms[m]]))
+                                for m in range(len(tms))},
+                "orders": "0: block = job; 1: one contiguous run of jobs per XCD"}
+        log(f"autotune {tune}")
 
     def step():
         batch.render(stream_src, outs, lut_list, stream)
@@ -344,6 +352,7 @@ def run_batched(args):
     line["single_stream"] = single
     line["batched_eager"] = eager
     line["batched_graph"] = graph
+    line["autotune"] = tune
     line["plain_dropin"] = plain
     line["plain_dropin_f32"] = plain32
     if plain:
@@ -632,6 +641,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()

@@ -264,6 +264,20 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders);
 int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
                       const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
                       hipStream_t stream);
+/* Block -> tile order of each stage's launch (remap, down, blend, final):
+ * 0 in job order, 1 one contiguous run of jobs per XCD (each XCD has its own
+ * L2, and neighbouring tiles share halos).  No reference counterpart: the
+ * reference launches one camera at a time.  The order changes no result. */
+#define PANO_SCHED_MODES 2
+int pano_batch_set_sched(pano_batch* b, const int* sched4);
+int pano_batch_get_sched(const pano_batch* b, int* sched4);
+/* Autotuner: renders the given frames iters times under each order, keeps the
+ * fastest order per stage (device time from events), and returns the mean ms
+ * per stage and order in ms_out (PANO_SCHED_MODES x 4, may be NULL).  The
+ * outputs are written as by pano_batch_render; a captured graph is dropped. */
+int pano_batch_autotune(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                        const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                        hipStream_t stream, int iters, double* ms_out);
 int pano_batch_set_timing(pano_batch* b, int enable);
 int pano_batch_stage_ms(pano_batch* b, double* ms4, int* frames);
 /* The per-frame sequence of one pano_batch_render as a HIP graph (no

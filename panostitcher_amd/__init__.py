@@ -119,6 +119,9 @@ def lib():
             "pano_batch_stage_ms": ([vp, vp, C.POINTER(C.c_int)], i),
             "pano_batch_capture": ([vp, pp, psz, pp, pp, psz, vp], i),
             "pano_batch_replay": ([vp, pp, vp], i),
+            "pano_batch_set_sched": ([vp, vp], i),
+            "pano_batch_get_sched": ([vp, vp], i),
+            "pano_batch_autotune": ([vp, pp, psz, pp, pp, psz, vp, i, vp], i),
         }
         for name, (args, res) in sigs.items():
             f = getattr(L, name)
@@ -145,7 +148,8 @@ def exported_symbols():
         "pano_render_fused", "pano_render_render_autogain", "pano_render_gains",
         "pano_render_gain_sums", "pano_batch_create", "pano_batch_destroy", "pano_batch_set",
         "pano_batch_render", "pano_batch_set_timing", "pano_batch_stage_ms",
-        "pano_batch_capture", "pano_batch_replay",
+        "pano_batch_capture", "pano_batch_replay", "pano_batch_set_sched", "pano_batch_get_sched",
+        "pano_batch_autotune",
         "pano_server_create", "pano_server_destroy", "pano_server_admit", "pano_server_retire",
         "pano_server_submit", "pano_server_poll", "pano_server_drain", "pano_server_stats",
         "pano_blender32f_create", "pano_blender32f_destroy", "pano_blender32f_prepare",
@@ -646,6 +650,31 @@ class Batch:
         keep = []
         lp = self._lut_array(luts_per_render, keep)
         _check(lib().pano_batch_replay(self._h, lp, _stream(stream)), "batch replay")
+
+    SCHED_MODES = 2
+
+    def set_sched(self, sched4):
+        a = np.ascontiguousarray(sched4, np.int32)
+        _check(lib().pano_batch_set_sched(self._h, C.c_void_p(a.ctypes.data)), "set sched")
+
+    def sched(self):
+        a = np.zeros(4, np.int32)
+        _check(lib().pano_batch_get_sched(self._h, C.c_void_p(a.ctypes.data)), "get sched")
+        return a.tolist()
+
+    def autotune(self, srcs_per_render, outs, luts_per_render=None, stream=None, iters=5):
+        """Times every block order per stage on these frames and keeps the fastest
+        (pano_batch_autotune).  Returns (chosen orders, ms[mode][stage])."""
+        flat = [t for srcs in srcs_per_render for t in srcs]
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in flat])
+        optrs, osteps = zip(*[_dev_ptr_step(t) for t in outs])
+        keep = []
+        lp = self._lut_array(luts_per_render, keep)
+        ms = np.zeros((self.SCHED_MODES, 4), np.float64)
+        _check(lib().pano_batch_autotune(self._h, _ptrs(ptrs), _sizes(steps), lp, _ptrs(optrs),
+                                         _sizes(osteps), _stream(stream), int(iters),
+                                         C.c_void_p(ms.ctypes.data)), "batch autotune")
+        return self.sched(), ms
 
 
 class ServerDone(C.Structure):

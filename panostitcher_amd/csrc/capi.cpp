@@ -189,6 +189,7 @@ T* upload_vec(Arena& a, const std::vector<T>& v, int& err)
 struct Launch {
     int stage, level, nseg, blocks;
     const Seg* segs;
+    int sched = 0;   // block -> job order (block_job)
 };
 
 // The gain LUT pass ahead of a fused sequence: jobs over the frames of src.
@@ -214,7 +215,7 @@ int run_sequence(const std::vector<Launch>& seq, const SrcTable& st, const OutTa
             for (int k = prev + 1; k <= l.stage; k++) HIP_TRY(hipEventRecord((*ev)[k], s));
             prev = l.stage;
         }
-        HIP_TRY(launch_seg(l.stage, l.segs, l.nseg, l.blocks, l.level, st, ot, s));
+        HIP_TRY(launch_seg(l.stage, l.segs, l.nseg, l.blocks, l.level, st, ot, l.sched, s));
     }
     if (ev)
         for (int k = prev + 1; k <= 4; k++) HIP_TRY(hipEventRecord((*ev)[k], s));
@@ -799,6 +800,11 @@ struct pano_batch {
     int cur = -1;
     PinnedRing ring;
     StageTimer timer;
+    int sched[4] = {0, 0, 0, 0};        // per stage: block -> job order (autotuned)
+    void apply_sched()
+    {
+        for (Launch& l : seq) l.sched = sched[l.stage];
+    }
     // captured launch sequence (pano_batch_capture) and the tables it reads
     hipGraphExec_t exec = nullptr;
     int exec_slot = -1;
@@ -1476,6 +1482,7 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
     if (sizeof(Seg) * flat.size() + 256 + sizeof(LutJob) * lj.size() > pano_batch::kSlotBytes)
         return PANO_ERR_UNSUPPORTED;
     b->drop_graph();
+    for (Launch& l : ls) l.sched = b->sched[l.stage];
     b->segs.swap(flat);
     b->seg_off.swap(offs);
     b->lutjobs.swap(lj);
@@ -1624,6 +1631,65 @@ int pano_batch_replay(pano_batch* b, const uint8_t* const* luts, hipStream_t str
     pano_batch::Slot& sl = b->slots[b->cur];
     HIP_TRY(hipEventRecord(sl.ev, stream));
     sl.used = true;
+    return PANO_OK;
+}
+
+int pano_batch_set_sched(pano_batch* b, const int* sched4)
+{
+    if (!b || !sched4) return PANO_ERR_ARG;
+    for (int k = 0; k < 4; k++)
+        if (sched4[k] < 0 || sched4[k] >= kSchedModes) return PANO_ERR_ARG;
+    b->drop_graph();
+    for (int k = 0; k < 4; k++) b->sched[k] = sched4[k];
+    b->apply_sched();
+    return PANO_OK;
+}
+
+int pano_batch_get_sched(const pano_batch* b, int* sched4)
+{
+    if (!b || !sched4) return PANO_ERR_ARG;
+    for (int k = 0; k < 4; k++) sched4[k] = b->sched[k];
+    return PANO_OK;
+}
+
+int pano_batch_autotune(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                        const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                        hipStream_t stream, int iters, double* ms_out)
+{
+    if (!b || iters <= 0 || iters > 1000) return PANO_ERR_ARG;
+    if (b->renders.empty()) return PANO_ERR_STATE;
+    b->drop_graph();
+    const bool was_on = b->timer.on;
+    double scratch[4];
+    int frames = 0;
+    if (was_on) {
+        int e = b->timer.collect(scratch, &frames);   // the caller's pending frames are dropped
+        if (e) return e;
+    }
+    double ms[kSchedModes][4] = {};
+    int e = PANO_OK;
+    for (int m = 0; m < kSchedModes && !e; m++) {
+        for (int k = 0; k < 4; k++) b->sched[k] = m;
+        b->apply_sched();
+        b->timer.on = false;
+        e = pano_batch_render(b, srcs, src_steps, luts, dsts, dst_steps, stream);   // warm-up
+        b->timer.on = true;
+        for (int it = 0; it < iters && !e; it++)
+            e = pano_batch_render(b, srcs, src_steps, luts, dsts, dst_steps, stream);
+        if (!e) e = b->timer.collect(ms[m], &frames);
+    }
+    b->timer.on = was_on;
+    if (e) return e;
+    for (int k = 0; k < 4; k++) {
+        int best = 0;
+        for (int m = 1; m < kSchedModes; m++)
+            if (ms[m][k] < ms[best][k]) best = m;
+        b->sched[k] = best;
+    }
+    b->apply_sched();
+    if (ms_out)
+        for (int m = 0; m < kSchedModes; m++)
+            for (int k = 0; k < 4; k++) ms_out[m * 4 + k] = ms[m][k] / iters;
     return PANO_OK;
 }
 

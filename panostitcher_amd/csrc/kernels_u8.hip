@@ -173,13 +173,14 @@ __device__ __forceinline__ int lane(uint32_t v, int c) { return (int)((v >> (8 *
 // ------------------------------------------------------------ level 0 -> 1 --
 
 __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ segs, int nseg,
-                                                     SrcTable src)
+                                                     SrcTable src, int sched)
 {
     __shared__ uint32_t in[kIH * kIW];
     __shared__ uint2 hs[kIH * kFTW];
-    const Seg& S = seg_of(segs, nseg, blockIdx.x);
+    const int bid = block_job(sched);
+    const Seg& S = seg_of(segs, nseg, bid);
     const RenderDesc R = ldc(S.rd);
-    const DownJob J = ldc((const DownJob*)S.jobs + (blockIdx.x - S.begin));
+    const DownJob J = ldc((const DownJob*)S.jobs + (bid - S.begin));
     const RemapCam cam = ldc(R.cams + J.src);
     const Plane P0 = ldc(R.planes + cam.plane);
     const Plane D = ldc(R.planes + J.dst);
@@ -308,12 +309,13 @@ __global__ __launch_bounds__(256) void k_lut_src(const LutJob* __restrict__ jobs
 
 // ------------------------------------------------------- level j -> j+1 --
 
-__global__ __launch_bounds__(kDT) void k_down_u8(const Seg* __restrict__ segs, int nseg)
+__global__ __launch_bounds__(kDT) void k_down_u8(const Seg* __restrict__ segs, int nseg, int sched)
 {
     __shared__ uint32_t in[kIH * kIW];
     __shared__ uint2 hs[kIH * kFTW];
-    const Seg& Sg = seg_of(segs, nseg, blockIdx.x);
-    const DownJob J = ldc((const DownJob*)Sg.jobs + (blockIdx.x - Sg.begin));
+    const int bid = block_job(sched);
+    const Seg& Sg = seg_of(segs, nseg, bid);
+    const DownJob J = ldc((const DownJob*)Sg.jobs + (bid - Sg.begin));
     const Plane* planes = ldc(&Sg.rd->planes);
     const Plane S = ldc(planes + J.src);
     const Plane D = ldc(planes + J.dst);
@@ -341,13 +343,14 @@ __global__ __launch_bounds__(kDT) void k_down_u8(const Seg* __restrict__ segs, i
 
 constexpr int kBlendStage = 4;   // cameras of a blend tile staged at once
 
-__global__ __launch_bounds__(kQT) void k_blend_u8(const Seg* __restrict__ segs, int nseg, int level)
+__global__ __launch_bounds__(kQT) void k_blend_u8(const Seg* __restrict__ segs, int nseg, int level, int sched)
 {
     __shared__ uint32_t gs[kBlendStage][kSH * kSW];
     __shared__ int4 cs[kSH * kSW];
-    const Seg& Sg = seg_of(segs, nseg, blockIdx.x);
+    const int bid = block_job(sched);
+    const Seg& Sg = seg_of(segs, nseg, bid);
     const RenderDesc R = ldc(Sg.rd);
-    const TileJob J = ldc((const TileJob*)Sg.jobs + (blockIdx.x - Sg.begin));
+    const TileJob J = ldc((const TileJob*)Sg.jobs + (bid - Sg.begin));
     const bool top = J.flags & kTileTop;
     const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
     const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
@@ -460,13 +463,14 @@ __global__ __launch_bounds__(kQT) void k_blend_u8(const Seg* __restrict__ segs, 
 constexpr int kFinStage = 4;   // owner cameras staged at once (a tile rarely has more than 2)
 
 __global__ __launch_bounds__(kQT) void k_final_u8(const Seg* __restrict__ segs, int nseg,
-                                                  OutTable outs)
+                                                  OutTable outs, int sched)
 {
     __shared__ uint32_t gs[kFinStage][kSH * kSW];
     __shared__ int4 cs[kSH * kSW];
-    const Seg& Sg = seg_of(segs, nseg, blockIdx.x);
+    const int bid = block_job(sched);
+    const Seg& Sg = seg_of(segs, nseg, bid);
     const RenderDesc R = ldc(Sg.rd);
-    const TileJob J = ldc((const TileJob*)Sg.jobs + (blockIdx.x - Sg.begin));
+    const TileJob J = ldc((const TileJob*)Sg.jobs + (bid - Sg.begin));
     const int L = R.levels;
     const int qx = threadIdx.x % kQTW, qy = threadIdx.x / kQTW;
     const int X = J.X + 2 * qx, Y = J.Y + 2 * qy;
@@ -580,21 +584,21 @@ __global__ __launch_bounds__(kQT) void k_final_u8(const Seg* __restrict__ segs, 
 // ------------------------------------------------------------- launchers --
 
 hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int level,
-                      const SrcTable& src, const OutTable& outs, hipStream_t s)
+                      const SrcTable& src, const OutTable& outs, int sched, hipStream_t s)
 {
     if (blocks <= 0 || nseg <= 0) return hipSuccess;
     switch (stage) {
     case kStageRemapDown0:
-        hipLaunchKernelGGL(k_remap_down0, dim3(blocks), dim3(kDT), 0, s, segs, nseg, src);
+        hipLaunchKernelGGL(k_remap_down0, dim3(blocks), dim3(kDT), 0, s, segs, nseg, src, sched);
         break;
     case kStageDown:
-        hipLaunchKernelGGL(k_down_u8, dim3(blocks), dim3(kDT), 0, s, segs, nseg);
+        hipLaunchKernelGGL(k_down_u8, dim3(blocks), dim3(kDT), 0, s, segs, nseg, sched);
         break;
     case kStageBlend:
-        hipLaunchKernelGGL(k_blend_u8, dim3(blocks), dim3(kQT), 0, s, segs, nseg, level);
+        hipLaunchKernelGGL(k_blend_u8, dim3(blocks), dim3(kQT), 0, s, segs, nseg, level, sched);
         break;
     case kStageFinal:
-        hipLaunchKernelGGL(k_final_u8, dim3(blocks), dim3(kQT), 0, s, segs, nseg, outs);
+        hipLaunchKernelGGL(k_final_u8, dim3(blocks), dim3(kQT), 0, s, segs, nseg, outs, sched);
         break;
     default:
         return hipErrorInvalidValue;

@@ -112,6 +112,7 @@ struct Seg {
 };
 
 constexpr int kMaxBatch = 32;   // renders per batched launch
+constexpr int kSchedModes = 2;  // block -> job orders of the batched kernels (block_job)
 
 // Per camera of the render path (resident).  map/mask cover the camera's
 // level-0 box.  For the 8-bit fused path they carry a kMapMargin frame of

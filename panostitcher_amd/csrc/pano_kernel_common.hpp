@@ -258,6 +258,23 @@ __device__ __forceinline__ void tap_offsets(uint32_t sstep, int sw, int sh, floa
     o[3] = o3;
 }
 
+// The job a block of a batched launch runs.  Workgroups are dispatched to the
+// 8 XCDs round-robin (block b to XCD b mod 8), each XCD with its own L2.
+// sched 0: job = block.  sched 1: each XCD gets one contiguous run of jobs, so
+// neighbouring tiles, whose staged halos overlap, share an L2.  Both are
+// bijections on [0, gridDim.x).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int block_job(int sched)
+{
+    const int b = blockIdx.x;
+    if (sched == 0) return b;
+    const int B = gridDim.x;
+    const int per = (B + kXcds - 1) / kXcds;   // jobs of the first `tall` XCDs
+    const int tall = B - kXcds * (per - 1);    // 1 .. kXcds
+    const int x = b % kXcds, l = b / kXcds;
+    return x < tall ? x * per + l : tall * per + (x - tall) * (per - 1) + l;
+}
+
 // Segment of a batched launch that owns block b (segments are few and ordered).
 __device__ __forceinline__ const Seg& seg_of(const Seg* segs, int nseg, int b)
 {

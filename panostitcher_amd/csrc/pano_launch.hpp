@@ -39,8 +39,9 @@ hipError_t launch_transform(const uint8_t* src, size_t src_step, uint8_t* dst, s
 // Fused 8-bit path (kernels_u8.hip): one launch of a stage over the segments
 // of a batch (blocks = sum of the segments' job counts).
 enum Stage : int { kStageRemapDown0 = 0, kStageDown = 1, kStageBlend = 2, kStageFinal = 3 };
+// sched: block -> job order (block_job): 0 in order, 1 contiguous per XCD.
 hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int level,
-                      const SrcTable& src, const OutTable& outs, hipStream_t s);
+                      const SrcTable& src, const OutTable& outs, int sched, hipStream_t s);
 
 // Gain LUT pass over source frames ahead of k_remap_down0 (blocks = the
 // jobs' block total; src holds the frames the jobs' src fields index).

@@ -325,6 +325,26 @@ def test_batch_matches_alone_and_oracle():
     b.render([srcs[2], srcs[0]], o2, [luts[2], luts[0]])
     np.testing.assert_array_equal(from_dev(o2[0]), alone[2])
     np.testing.assert_array_equal(from_dev(o2[1]), alone[0])
+    # every block -> tile order of every stage gives the same canvases, and so
+    # does the autotuner's choice
+    b.set(renders)
+    for sched in ([1, 1, 1, 1], [1, 0, 1, 0], [0, 1, 0, 1]):
+        note(f"batch block order {sched}")
+        b.set_sched(sched)
+        assert b.sched() == sched
+        for o in outs:
+            o.zero_()
+        b.render(srcs, outs, luts)
+        for k, o in enumerate(outs):
+            np.testing.assert_array_equal(from_dev(o), alone[k])
+    note("batch autotune")
+    chosen, ms = b.autotune(srcs, outs, luts, iters=2)
+    assert all(c in (0, 1) for c in chosen) and ms.shape == (2, 4) and (ms[:, 0] > 0).all()
+    for o in outs:
+        o.zero_()
+    b.render(srcs, outs, luts)
+    for k, o in enumerate(outs):
+        np.testing.assert_array_equal(from_dev(o), alone[k])
 
 
 def test_server_results_same_alone_and_in_the_mix():
