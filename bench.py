@@ -276,7 +276,7 @@ def run_batched(args):
     if not args.no_autotune:
         chosen, tms = batch.autotune(stream_src, outs, lut_list, stream, iters=3)
         tune = {"chosen": dict(zip(STAGES, chosen)),
-                "ms_per_step": {f"order{m}": dict(zip(STAGES, [round(v, 4) for v in tms[m]]))
+                "ms_per_step": {f"order{m}": dict(zip(STAGES, [round(float(v), 4) for v in tms[m]]))
                                 for m in range(len(tms))},
                 "orders": "0: block = job; 1: one contiguous run of jobs per XCD"}
         log(f"autotune {tune}")
@@ -319,6 +319,27 @@ def run_batched(args):
         if dtg < elapsed:
             elapsed = dtg
 
+    # the encoder hand-off: NV12 planes written by the level-0 kernel, against
+    # the BGRx canvas followed by the standalone conversion (cvtBGR32ToNV12)
+    enc = None
+    if not args.no_encoder:
+        yps = [torch.empty((wl.dst_h, wl.dst_w), dtype=torch.uint8, device="cuda") for _ in range(S)]
+        uvs = [torch.empty((wl.dst_h // 2, wl.dst_w), dtype=torch.uint8, device="cuda") for _ in range(S)]
+        nv12 = [P.Output.nv12(y, uv) for y, uv in zip(yps, uvs)]
+        dtn = timed(lambda: batch.render_to(stream_src, nv12, lut_list, stream), args.steps, args.warmup)
+
+        def then_convert():
+            batch.render(stream_src, outs, lut_list, stream)
+            for k in range(S):
+                P.cvt_bgr32_to_nv12(outs[k], yps[k], uvs[k], stream)
+
+        dtc = timed(then_convert, max(3, args.steps // 2), 1)
+        enc = {"nv12_fused": {"value": round(S * wl.out_pixels * args.steps / dtn / 1e6, 2), "unit": "Mpix/s",
+                              "ms_per_step": round(dtn * 1e3 / args.steps, 4)},
+               "bgrx_then_nv12": {"value": round(S * wl.out_pixels * max(3, args.steps // 2) / dtc / 1e6, 2),
+                                  "unit": "Mpix/s", "ms_per_step": round(dtc * 1e3 / max(3, args.steps // 2), 4)}}
+        log(f"encoder hand-off {enc}")
+
     ms_per_step = elapsed * 1e3 / args.steps
     value = S * wl.out_pixels * args.steps / elapsed / 1e6
     per_frame = {k: v / S for k, v in per_step.items()}
@@ -353,6 +374,7 @@ def run_batched(args):
     line["batched_eager"] = eager
     line["batched_graph"] = graph
     line["autotune"] = tune
+    line["encoder_handoff"] = enc
     line["plain_dropin"] = plain
     line["plain_dropin_f32"] = plain32
     if plain:
@@ -642,6 +664,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-encoder", action="store_true")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run a plain device copy of this many MiB (counter calibration)")
     args = ap.parse_args()

@@ -50,6 +50,21 @@ extern "C" {
 #define PANO_16SC4 27
 #define PANO_32FC4 29
 
+/* ---- output of a render: the BGRx canvas or the encoder's planes --------
+ * PANO_OUT_NV12 / PANO_OUT_YUV420P write what cvtBGR32ToNV12 /
+ * cvtBGR32ToYUV420P (CudaAccel/CudaInterface.h:161-166) would make of the
+ * BGRx canvas, straight from the level-0 kernel.  plane / step: BGRX {canvas};
+ * NV12 {Y, interleaved UV}; YUV420P {Y, U, V}.  Y and UV planes need even
+ * steps and 2-byte aligned pointers. */
+#define PANO_OUT_BGRX 0
+#define PANO_OUT_NV12 1
+#define PANO_OUT_YUV420P 2
+typedef struct pano_output {
+    int format;
+    uint8_t* plane[3];
+    size_t step[3];
+} pano_output;
+
 const char* pano_status_string(int status);
 int pano_version(void);
 
@@ -166,6 +181,20 @@ int pano_reproject(const uint8_t* src, size_t src_step, int src_w, int src_h, co
 int pano_transform_lut3(const uint8_t* src, size_t src_step, uint8_t* dst, size_t dst_step, int w,
                         int h, const uint8_t* luts, hipStream_t stream);
 
+/* Encoder / decoder hand-off: cvtBGR32ToYUV420P, cvtBGR32ToNV12,
+ * cvtYUV420PToBGR32, cvtNV12ToBGR32 (CudaAccel/CudaInterface.h:161-171,
+ * CudaAccel/ImageProc.cu:44-266), BT.601 studio swing in 20-bit fixed point.
+ * w and h even; BGRx rows 8-byte aligned; Y (and NV12 UV) rows 2-byte aligned.
+ * The BGRx outputs of the inverses have channel 3 = 0. */
+int pano_bgr32_to_yuv420p(const uint8_t* src, size_t src_step, int w, int h, uint8_t* y, size_t y_step,
+                          uint8_t* u, size_t u_step, uint8_t* v, size_t v_step, hipStream_t stream);
+int pano_bgr32_to_nv12(const uint8_t* src, size_t src_step, int w, int h, uint8_t* y, size_t y_step,
+                       uint8_t* uv, size_t uv_step, hipStream_t stream);
+int pano_yuv420p_to_bgr32(const uint8_t* y, size_t y_step, const uint8_t* u, size_t u_step, const uint8_t* v,
+                          size_t v_step, int w, int h, uint8_t* dst, size_t dst_step, hipStream_t stream);
+int pano_nv12_to_bgr32(const uint8_t* y, size_t y_step, const uint8_t* uv, size_t uv_step, int w, int h,
+                       uint8_t* dst, size_t dst_step, hipStream_t stream);
+
 /* getLUT (Blend/ExposureColorOptimize.cpp:1217-1223): lut[v] = sat_u8(rint(k v)). */
 int pano_get_lut(uint8_t* lut, double k);
 /* getExposureColorOptimizeLUTs (ExposureColorOptimize.cpp:1425-1443) without
@@ -208,6 +237,10 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
 int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
                        const size_t* src_steps, const uint8_t* luts, uint8_t* dst,
                        size_t dst_step, hipStream_t stream);
+/* The same frame written as `out` (the canvas, or the encoder's NV12 / YUV420P
+ * planes that the tasks produce next, PanoramaLocalDiskTask.cpp:2053-2071). */
+int pano_render_render_to(pano_render* r, int num, const uint8_t* const* srcs, const size_t* src_steps,
+                          const uint8_t* luts, const pano_output* out, hipStream_t stream);
 
 int pano_render_num_images(const pano_render* r);
 int pano_render_num_levels(const pano_render* r);
@@ -264,6 +297,9 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders);
 int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
                       const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
                       hipStream_t stream);
+/* The same with one pano_output per render. */
+int pano_batch_render_to(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                         const uint8_t* const* luts, const pano_output* outs, hipStream_t stream);
 /* Block -> tile order of each stage's launch (remap, down, blend, final):
  * 0 in job order, 1 one contiguous run of jobs per XCD (each XCD has its own
  * L2, and neighbouring tiles share halos).  No reference counterpart: the
@@ -291,6 +327,8 @@ int pano_batch_stage_ms(pano_batch* b, double* ms4, int* frames);
 int pano_batch_capture(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
                        const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
                        hipStream_t stream);
+int pano_batch_capture_to(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                          const uint8_t* const* luts, const pano_output* outs, hipStream_t stream);
 int pano_batch_replay(pano_batch* b, const uint8_t* const* luts, hipStream_t stream);
 
 /* ---- serving loop ----------------------------------------------------------

@@ -136,6 +136,14 @@ void orc_pyrup32f(const float* src, int sw, int sh, int cn, float* dst, int dw, 
 int orc_blend32f(int n, int rows, int cols, const uint8_t* masks, const float* images,
                  int max_levels, int min_length, uint8_t* out);
 
+/* Encoder / decoder hand-off (yuv.c): CudaAccel/ImageProc.cu.  nv12: u is the
+ * interleaved UV plane (w bytes per row, v unused); else u and v are w/2 x h/2.
+ * y is w x h; w and h even.  Return 0 on bad sizes. */
+int orc_bgr32_to_yuv(const uint8_t* bgr, size_t step, int w, int h, int nv12, uint8_t* y,
+                     uint8_t* u, uint8_t* v);
+int orc_yuv_to_bgr32(const uint8_t* y, const uint8_t* u, const uint8_t* v, int w, int h, int nv12,
+                     uint8_t* bgr, size_t step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,10 @@ def lib():
         L.orc_blender_prepare.restype = i
         L.orc_blender_blend.argtypes = [vp, vp, vp, i]
         L.orc_blender_blend.restype = i
+        L.orc_bgr32_to_yuv.argtypes = [vp, C.c_size_t, i, i, i, vp, vp, vp]
+        L.orc_bgr32_to_yuv.restype = i
+        L.orc_yuv_to_bgr32.argtypes = [vp, vp, vp, i, i, i, vp, C.c_size_t]
+        L.orc_yuv_to_bgr32.restype = i
         L.orc_blender_blend_custom.argtypes = [vp, vp, vp, vp, i]
         L.orc_blender_blend_custom.restype = i
         L.orc_blender_num_levels.argtypes = [vp]
@@ -345,6 +349,45 @@ def blend32f(masks: np.ndarray, images: np.ndarray, max_levels: int, min_length:
     out = np.zeros((h, w, 4), np.uint8)
     L = lib().orc_blend32f(n, h, w, _p(masks), _p(images), max_levels, min_length, _p(out))
     return out, L
+
+
+def bgr32_to_yuv420p(bgrx: np.ndarray):
+    """cvtBGR32ToYUV420P (ImageProc.cu:44-141): (h, w, 4) u8 -> y, u, v planes."""
+    bgrx = np.ascontiguousarray(bgrx, np.uint8)
+    h, w = bgrx.shape[:2]
+    y = np.zeros((h, w), np.uint8)
+    u = np.zeros((h // 2, w // 2), np.uint8)
+    v = np.zeros((h // 2, w // 2), np.uint8)
+    assert lib().orc_bgr32_to_yuv(_p(bgrx), w * 4, w, h, 0, _p(y), _p(u), _p(v))
+    return y, u, v
+
+
+def bgr32_to_nv12(bgrx: np.ndarray):
+    """cvtBGR32ToNV12 (ImageProc.cu:123-159): (h, w, 4) u8 -> y, interleaved uv."""
+    bgrx = np.ascontiguousarray(bgrx, np.uint8)
+    h, w = bgrx.shape[:2]
+    y = np.zeros((h, w), np.uint8)
+    uv = np.zeros((h // 2, w), np.uint8)
+    assert lib().orc_bgr32_to_yuv(_p(bgrx), w * 4, w, h, 1, _p(y), _p(uv), None)
+    return y, uv
+
+
+def yuv420p_to_bgr32(y, u, v):
+    """cvtYUV420PToBGR32 (ImageProc.cu:197-242); channel 3 of the output is 0."""
+    y, u, v = (np.ascontiguousarray(a, np.uint8) for a in (y, u, v))
+    h, w = y.shape
+    out = np.zeros((h, w, 4), np.uint8)
+    assert lib().orc_yuv_to_bgr32(_p(y), _p(u), _p(v), w, h, 0, _p(out), w * 4)
+    return out
+
+
+def nv12_to_bgr32(y, uv):
+    """cvtNV12ToBGR32 (ImageProc.cu:211-258); channel 3 of the output is 0."""
+    y, uv = (np.ascontiguousarray(a, np.uint8) for a in (y, uv))
+    h, w = y.shape
+    out = np.zeros((h, w, 4), np.uint8)
+    assert lib().orc_yuv_to_bgr32(_p(y), _p(uv), None, w, h, 1, _p(out), w * 4)
+    return out
 
 
 class PhotoParam(C.Structure):

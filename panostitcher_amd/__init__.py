@@ -33,6 +33,7 @@ PANO_ERR_BUDGET = -6
 PANO_8UC4 = 24
 PANO_16SC4 = 27
 PANO_32FC4 = 29
+PANO_OUT_BGRX, PANO_OUT_NV12, PANO_OUT_YUV420P = 0, 1, 2
 
 _lib = None
 
@@ -93,6 +94,13 @@ def lib():
             "pano_render_prepare": ([vp, i, vp, vp, vp, i, i, i], i),
             "pano_render_prepare_maps": ([vp, i, vp, vp, i, i, pp, pp, pp, i], i),
             "pano_render_render": ([vp, i, pp, psz, vp, vp, sz, vp], i),
+            "pano_render_render_to": ([vp, i, pp, psz, vp, vp, vp], i),
+            "pano_batch_render_to": ([vp, pp, psz, pp, vp, vp], i),
+            "pano_batch_capture_to": ([vp, pp, psz, pp, vp, vp], i),
+            "pano_bgr32_to_yuv420p": ([vp, sz, i, i, vp, sz, vp, sz, vp, sz, vp], i),
+            "pano_bgr32_to_nv12": ([vp, sz, i, i, vp, sz, vp, sz, vp], i),
+            "pano_yuv420p_to_bgr32": ([vp, sz, vp, sz, vp, sz, i, i, vp, sz, vp], i),
+            "pano_nv12_to_bgr32": ([vp, sz, vp, sz, i, i, vp, sz, vp], i),
             "pano_render_num_images": ([vp], i),
             "pano_render_num_levels": ([vp], i),
             "pano_render_mem_bytes": ([vp], i64),
@@ -154,7 +162,9 @@ def exported_symbols():
         "pano_server_submit", "pano_server_poll", "pano_server_drain", "pano_server_stats",
         "pano_blender32f_create", "pano_blender32f_destroy", "pano_blender32f_prepare",
         "pano_blender32f_blend", "pano_blender32f_num_levels", "pano_reproject",
-        "pano_blender_blend_custom",
+        "pano_blender_blend_custom", "pano_render_render_to", "pano_batch_render_to",
+        "pano_batch_capture_to", "pano_bgr32_to_yuv420p", "pano_bgr32_to_nv12",
+        "pano_yuv420p_to_bgr32", "pano_nv12_to_bgr32",
     ]
 
 
@@ -181,6 +191,35 @@ def _stream(stream):
 
         stream = torch.cuda.current_stream()
     return C.c_void_p(stream.cuda_stream)
+
+
+class Output(C.Structure):
+    """pano_output: a render's destination.  Output.bgrx(canvas), Output.nv12(y, uv)
+    or Output.yuv420p(y, u, v) over uint8 cuda tensors (rows may be padded)."""
+    _fields_ = [("format", C.c_int), ("plane", C.c_void_p * 3), ("step", C.c_size_t * 3)]
+
+    @classmethod
+    def _make(cls, fmt, planes):
+        o = cls()
+        o.format = fmt
+        for k, t in enumerate(planes):
+            p, st = _dev_ptr_step(t)
+            o.plane[k] = p
+            o.step[k] = st
+        o._tensors = list(planes)
+        return o
+
+    @classmethod
+    def bgrx(cls, canvas):
+        return cls._make(PANO_OUT_BGRX, [canvas])
+
+    @classmethod
+    def nv12(cls, y, uv):
+        return cls._make(PANO_OUT_NV12, [y, uv])
+
+    @classmethod
+    def yuv420p(cls, y, u, v):
+        return cls._make(PANO_OUT_YUV420P, [y, u, v])
 
 
 def _dev_ptr_step(t):
@@ -439,6 +478,68 @@ def cuda_reproject(src, xmap, ymap, dst=None, stream=None):
     return dst
 
 
+def _planes_yuv(h, w, nv12, device):
+    import torch
+
+    y = torch.empty((h, w), dtype=torch.uint8, device=device)
+    if nv12:
+        return y, torch.empty((h // 2, w), dtype=torch.uint8, device=device)
+    return (y, torch.empty((h // 2, w // 2), dtype=torch.uint8, device=device),
+            torch.empty((h // 2, w // 2), dtype=torch.uint8, device=device))
+
+
+def cvt_bgr32_to_nv12(src, y=None, uv=None, stream=None):
+    """cvtBGR32ToNV12(bgr32, y, uv) (CudaInterface.h:164-165)."""
+    h, w = src.shape[:2]
+    if y is None or uv is None:
+        y, uv = _planes_yuv(h, w, True, src.device)
+    sp, ss = _dev_ptr_step(src)
+    _check(lib().pano_bgr32_to_nv12(C.c_void_p(sp), ss, w, h, C.c_void_p(y.data_ptr()), y.stride(0),
+                                    C.c_void_p(uv.data_ptr()), uv.stride(0), _stream(stream)),
+           "cvtBGR32ToNV12")
+    return y, uv
+
+
+def cvt_bgr32_to_yuv420p(src, y=None, u=None, v=None, stream=None):
+    """cvtBGR32ToYUV420P(bgr32, y, u, v) (CudaInterface.h:161-162)."""
+    h, w = src.shape[:2]
+    if y is None or u is None or v is None:
+        y, u, v = _planes_yuv(h, w, False, src.device)
+    sp, ss = _dev_ptr_step(src)
+    _check(lib().pano_bgr32_to_yuv420p(C.c_void_p(sp), ss, w, h, C.c_void_p(y.data_ptr()), y.stride(0),
+                                       C.c_void_p(u.data_ptr()), u.stride(0), C.c_void_p(v.data_ptr()),
+                                       v.stride(0), _stream(stream)), "cvtBGR32ToYUV420P")
+    return y, u, v
+
+
+def cvt_nv12_to_bgr32(y, uv, dst=None, stream=None):
+    """cvtNV12ToBGR32(y, uv, bgr32) (CudaInterface.h:170-171); channel 3 = 0."""
+    import torch
+
+    h, w = y.shape
+    if dst is None:
+        dst = torch.empty((h, w, 4), dtype=torch.uint8, device=y.device)
+    dp, ds = _dev_ptr_step(dst)
+    _check(lib().pano_nv12_to_bgr32(C.c_void_p(y.data_ptr()), y.stride(0), C.c_void_p(uv.data_ptr()),
+                                    uv.stride(0), w, h, C.c_void_p(dp), ds, _stream(stream)),
+           "cvtNV12ToBGR32")
+    return dst
+
+
+def cvt_yuv420p_to_bgr32(y, u, v, dst=None, stream=None):
+    """cvtYUV420PToBGR32(y, u, v, bgr32) (CudaInterface.h:167-168); channel 3 = 0."""
+    import torch
+
+    h, w = y.shape
+    if dst is None:
+        dst = torch.empty((h, w, 4), dtype=torch.uint8, device=y.device)
+    dp, ds = _dev_ptr_step(dst)
+    _check(lib().pano_yuv420p_to_bgr32(C.c_void_p(y.data_ptr()), y.stride(0), C.c_void_p(u.data_ptr()),
+                                       u.stride(0), C.c_void_p(v.data_ptr()), v.stride(0), w, h,
+                                       C.c_void_p(dp), ds, _stream(stream)), "cvtYUV420PToBGR32")
+    return dst
+
+
 def cuda_transform(src, luts, dst=None, stream=None):
     """cudaTransform(src 8UC4, dst, luts 3x256)."""
     import torch
@@ -543,6 +644,21 @@ class PanoramaRender:
         _check(st, "render")
         return True
 
+    def render_to(self, srcs, output: "Output", luts=None, stream=None) -> bool:
+        """render() into an Output (the canvas, or NV12 / YUV420P planes)."""
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in srcs])
+        lp = None
+        if luts is not None:
+            luts = np.ascontiguousarray(luts, np.uint8)
+            assert luts.shape == (len(srcs), 3, 256)
+            lp = C.c_void_p(luts.ctypes.data)
+        st = lib().pano_render_render_to(self._h, len(srcs), _ptrs(ptrs), _sizes(steps), lp,
+                                         C.byref(output), _stream(stream))
+        if st == -2:
+            return False
+        _check(st, "render_to")
+        return True
+
     def render_autogain(self, srcs, out, tint=True, stream=None) -> bool:
         """render() with gains estimated from this frame's overlap means
         (GainCompensate / getTransformsMeanApproxSiftPanoPaper, optionally the
@@ -633,6 +749,27 @@ class Batch:
         _check(lib().pano_batch_render(self._h, _ptrs(ptrs), _sizes(steps), lp, _ptrs(optrs),
                                        _sizes(osteps), _stream(stream)), "batch render")
         return True
+
+    def render_to(self, srcs_per_render, outputs, luts_per_render=None, stream=None):
+        """render() with one Output per render."""
+        flat = [t for srcs in srcs_per_render for t in srcs]
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in flat])
+        keep = []
+        lp = self._lut_array(luts_per_render, keep)
+        arr = (Output * len(outputs))(*outputs)
+        _check(lib().pano_batch_render_to(self._h, _ptrs(ptrs), _sizes(steps), lp, arr,
+                                          _stream(stream)), "batch render_to")
+        return True
+
+    def capture_to(self, srcs_per_render, outputs, luts_per_render=None, stream=None):
+        flat = [t for srcs in srcs_per_render for t in srcs]
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in flat])
+        keep = []
+        lp = self._lut_array(luts_per_render, keep)
+        arr = (Output * len(outputs))(*outputs)
+        _check(lib().pano_batch_capture_to(self._h, _ptrs(ptrs), _sizes(steps), lp, arr,
+                                           _stream(stream)), "batch capture_to")
+        self._graph_buffers = (flat, list(outputs))
 
     def capture(self, srcs_per_render, outs, luts_per_render=None, stream=None):
         """Records one render of these buffers as a HIP graph (pano_batch_capture);

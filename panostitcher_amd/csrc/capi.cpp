@@ -185,6 +185,14 @@ T* upload_vec(Arena& a, const std::vector<T>& v, int& err)
     return d;
 }
 
+// One plane of an output: rows of at least `row` bytes, step a multiple of
+// `align` (as is the pointer), below 2^24, the plane below 4 GB.
+bool plane_ok(const uint8_t* p, size_t step, size_t row, int rows, size_t align)
+{
+    return p && step >= row && step % align == 0 && (uintptr_t)p % align == 0 && step < (1u << 24) &&
+           step * (uint64_t)rows <= 0xffffffffu;
+}
+
 // One launch of the fused 8-bit sequence: a stage over a device segment table.
 struct Launch {
     int stage, level, nseg, blocks;
@@ -753,6 +761,11 @@ struct pano_render {
     StageTimer timer;
     int lut_state = 0;                // the device's "apply LUTs" byte
     std::vector<uint8_t> lut_block;   // n x 768 LUTs + the flag byte
+    uint8_t* bgrx_scratch = nullptr;  // 16-bit path with encoder-plane output: the BGRx canvas
+    ~pano_render()
+    {
+        if (bgrx_scratch) (void)hipFree(bgrx_scratch);
+    }
 
     // Stream-ordered update of the LUT buffer for this frame (fused path).
     int upload_luts(const uint8_t* luts, hipStream_t s)
@@ -1100,6 +1113,48 @@ int pano_reproject(const uint8_t* src, size_t src_step, int src_w, int src_h, co
     return PANO_OK;
 }
 
+int pano_bgr32_to_yuv420p(const uint8_t* src, size_t src_step, int w, int h, uint8_t* y, size_t y_step,
+                          uint8_t* u, size_t u_step, uint8_t* v, size_t v_step, hipStream_t stream)
+{
+    if (w <= 0 || h <= 0 || (w | h) & 1 || !plane_ok(src, src_step, (size_t)w * 4, h, 8) ||
+        !plane_ok(y, y_step, w, h, 2) || !plane_ok(u, u_step, w / 2, h / 2, 1) ||
+        !plane_ok(v, v_step, w / 2, h / 2, 1))
+        return PANO_ERR_ARG;
+    HIP_TRY(launch_bgr32_to_yuv(false, src, src_step, w, h, y, y_step, u, u_step, v, v_step, stream));
+    return PANO_OK;
+}
+
+int pano_bgr32_to_nv12(const uint8_t* src, size_t src_step, int w, int h, uint8_t* y, size_t y_step,
+                       uint8_t* uv, size_t uv_step, hipStream_t stream)
+{
+    if (w <= 0 || h <= 0 || (w | h) & 1 || !plane_ok(src, src_step, (size_t)w * 4, h, 8) ||
+        !plane_ok(y, y_step, w, h, 2) || !plane_ok(uv, uv_step, w, h / 2, 2))
+        return PANO_ERR_ARG;
+    HIP_TRY(launch_bgr32_to_yuv(true, src, src_step, w, h, y, y_step, uv, uv_step, nullptr, 0, stream));
+    return PANO_OK;
+}
+
+int pano_yuv420p_to_bgr32(const uint8_t* y, size_t y_step, const uint8_t* u, size_t u_step, const uint8_t* v,
+                          size_t v_step, int w, int h, uint8_t* dst, size_t dst_step, hipStream_t stream)
+{
+    if (w <= 0 || h <= 0 || (w | h) & 1 || !plane_ok(dst, dst_step, (size_t)w * 4, h, 8) ||
+        !plane_ok(y, y_step, w, h, 2) || !plane_ok(u, u_step, w / 2, h / 2, 1) ||
+        !plane_ok(v, v_step, w / 2, h / 2, 1))
+        return PANO_ERR_ARG;
+    HIP_TRY(launch_yuv_to_bgr32(false, y, y_step, u, u_step, v, v_step, w, h, dst, dst_step, stream));
+    return PANO_OK;
+}
+
+int pano_nv12_to_bgr32(const uint8_t* y, size_t y_step, const uint8_t* uv, size_t uv_step, int w, int h,
+                       uint8_t* dst, size_t dst_step, hipStream_t stream)
+{
+    if (w <= 0 || h <= 0 || (w | h) & 1 || !plane_ok(dst, dst_step, (size_t)w * 4, h, 8) ||
+        !plane_ok(y, y_step, w, h, 2) || !plane_ok(uv, uv_step, w, h / 2, 2))
+        return PANO_ERR_ARG;
+    HIP_TRY(launch_yuv_to_bgr32(true, y, y_step, uv, uv_step, nullptr, 0, w, h, dst, dst_step, stream));
+    return PANO_OK;
+}
+
 int pano_transform_lut3(const uint8_t* src, size_t src_step, uint8_t* dst, size_t dst_step, int w,
                         int h, const uint8_t* luts, hipStream_t stream)
 {
@@ -1237,17 +1292,60 @@ int pano_render_prepare(pano_render* r, int num, const pano_photo_param* params,
 
 namespace {
 
+// Checks an output of a w x h canvas and enters it as OutTable entry k.
+int output_entry(const pano_output& o, int w, int h, OutTable& ot, int k)
+{
+    switch (o.format) {
+    case PANO_OUT_BGRX:
+        if (!o.plane[0] || o.step[0] < (size_t)w * 4 || o.step[0] % 4 || o.step[0] >= (1u << 24) ||
+            o.step[0] * (uint64_t)h > 0xffffffffu)
+            return PANO_ERR_ARG;
+        ot.fmt[k] = kOutBGRX;
+        break;
+    case PANO_OUT_NV12:
+        if (!plane_ok(o.plane[0], o.step[0], w, h, 2) || !plane_ok(o.plane[1], o.step[1], w, h / 2, 2))
+            return PANO_ERR_ARG;
+        ot.fmt[k] = kOutNV12;
+        break;
+    case PANO_OUT_YUV420P:
+        if (!plane_ok(o.plane[0], o.step[0], w, h, 2) || !plane_ok(o.plane[1], o.step[1], w / 2, h / 2, 1) ||
+            !plane_ok(o.plane[2], o.step[2], w / 2, h / 2, 1))
+            return PANO_ERR_ARG;
+        ot.fmt[k] = kOutYUV420P;
+        break;
+    default:
+        return PANO_ERR_ARG;
+    }
+    if ((w | h) & 1) return PANO_ERR_ARG;   // canvases are even (W = 2H, H even)
+    ot.p[k] = o.plane[0];
+    ot.step[k] = (uint32_t)o.step[0];
+    ot.p2[k] = o.plane[1];
+    ot.step2[k] = (uint32_t)o.step[1];
+    ot.p3[k] = o.plane[2];
+    ot.step3[k] = (uint32_t)o.step[2];
+    return PANO_OK;
+}
+
+pano_output bgrx_output(uint8_t* dst, size_t step)
+{
+    pano_output o{};
+    o.format = PANO_OUT_BGRX;
+    o.plane[0] = dst;
+    o.step[0] = step;
+    return o;
+}
+
 // Argument checks of a render call; fills the frame and output tables.
 int render_args(const pano_render* r, int num, const uint8_t* const* srcs, const size_t* src_steps,
-                uint8_t* dst, size_t dst_step, SrcTable& st, OutTable& ot)
+                const pano_output& out, SrcTable& st, OutTable& ot)
 {
     if (!r) return PANO_ERR_ARG;
     if (!r->ok) return PANO_ERR_STATE;
     const DeviceRig& d = *r->rig;
-    if (num != d.prep.n || !srcs || !src_steps || !dst) return PANO_ERR_ARG;
-    if (dst_step < (size_t)r->dst_w * 4 || dst_step % 4 || dst_step >= (1u << 24) ||
-        dst_step * (uint64_t)r->dst_h > 0xffffffffu)
-        return PANO_ERR_ARG;
+    if (num != d.prep.n || !srcs || !src_steps) return PANO_ERR_ARG;
+    ot = OutTable{};
+    int e = output_entry(out, r->dst_w, r->dst_h, ot, 0);
+    if (e) return e;
     st = SrcTable{};
     for (int i = 0; i < num; i++) {
         if (!srcs[i] || src_steps[i] < (size_t)r->src_w[i] * 4 || src_steps[i] % 4 ||
@@ -1256,9 +1354,6 @@ int render_args(const pano_render* r, int num, const uint8_t* const* srcs, const
         st.p[i] = srcs[i];
         st.step[i] = (uint32_t)src_steps[i];
     }
-    ot = OutTable{};
-    ot.p[0] = dst;
-    ot.step[0] = (uint32_t)dst_step;
     return PANO_OK;
 }
 
@@ -1283,15 +1378,30 @@ int run_fused_with_luts(DeviceRig& d, const SrcTable& st, const OutTable& ot, hi
 
 extern "C" {
 
-int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
-                       const size_t* src_steps, const uint8_t* luts, uint8_t* dst,
-                       size_t dst_step, hipStream_t stream)
+int pano_render_render_to(pano_render* r, int num, const uint8_t* const* srcs, const size_t* src_steps,
+                          const uint8_t* luts, const pano_output* out, hipStream_t stream)
 {
+    if (!out) return PANO_ERR_ARG;
     SrcTable st;
     OutTable ot;
-    int err = render_args(r, num, srcs, src_steps, dst, dst_step, st, ot);
+    int err = render_args(r, num, srcs, src_steps, *out, st, ot);
     if (err) return err;
     DeviceRig& d = *r->rig;
+    if (!d.prep.fused0 && out->format != PANO_OUT_BGRX) {
+        // the 16-bit kernels write BGRx: render into a resident canvas, then convert
+        const size_t step = (size_t)r->dst_w * 4;
+        if (!r->bgrx_scratch && hipMalloc((void**)&r->bgrx_scratch, step * r->dst_h) != hipSuccess) {
+            r->bgrx_scratch = nullptr;
+            return PANO_ERR_NOMEM;
+        }
+        const pano_output canvas = bgrx_output(r->bgrx_scratch, step);
+        err = pano_render_render_to(r, num, srcs, src_steps, luts, &canvas, stream);
+        if (err) return err;
+        const bool nv12 = out->format == PANO_OUT_NV12;
+        HIP_TRY(launch_bgr32_to_yuv(nv12, r->bgrx_scratch, step, r->dst_w, r->dst_h, out->plane[0], out->step[0],
+                                    out->plane[1], out->step[1], out->plane[2], out->step[2], stream));
+        return PANO_OK;
+    }
     std::vector<hipEvent_t>* ev = r->timer.begin_frame();
     if (d.prep.fused0) {
         int e = r->upload_luts(luts, stream);
@@ -1310,13 +1420,21 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
     return d.run_pyramid(-1, st, ot, stream, ev);
 }
 
+int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
+                       const size_t* src_steps, const uint8_t* luts, uint8_t* dst,
+                       size_t dst_step, hipStream_t stream)
+{
+    const pano_output o = bgrx_output(dst, dst_step);
+    return pano_render_render_to(r, num, srcs, src_steps, luts, &o, stream);
+}
+
 int pano_render_render_autogain(pano_render* r, int num, const uint8_t* const* srcs,
                                 const size_t* src_steps, int flags, uint8_t* dst,
                                 size_t dst_step, hipStream_t stream)
 {
     SrcTable st;
     OutTable ot;
-    int err = render_args(r, num, srcs, src_steps, dst, dst_step, st, ot);
+    int err = render_args(r, num, srcs, src_steps, bgrx_output(dst, dst_step), st, ot);
     if (err) return err;
     DeviceRig& d = *r->rig;
     if (!d.prep.fused0 || !d.d_cover || d.seq.empty()) return PANO_ERR_UNSUPPORTED;
@@ -1502,22 +1620,19 @@ namespace {
 // arguments; uploads the renders' LUTs and the batch's tables on the stream.
 // sr: the frames the remap reads (LUT-mapped copies for renders with LUTs).
 int batch_prepare(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
-                  const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
-                  hipStream_t stream, SrcTable& st, SrcTable& sr, OutTable& ot, bool& any)
+                  const uint8_t* const* luts, const pano_output* outs, hipStream_t stream, SrcTable& st,
+                  SrcTable& sr, OutTable& ot, bool& any)
 {
     if (!b) return PANO_ERR_ARG;
     if (b->renders.empty()) return PANO_ERR_STATE;
-    if (!srcs || !src_steps || !dsts || !dst_steps) return PANO_ERR_ARG;
+    if (!srcs || !src_steps || !outs) return PANO_ERR_ARG;
     st = SrcTable{};
     ot = OutTable{};
     int k = 0;
     for (size_t i = 0; i < b->renders.size(); i++) {
         pano_render* r = b->renders[i];
-        if (!dsts[i] || dst_steps[i] < (size_t)r->dst_w * 4 || dst_steps[i] % 4 ||
-            dst_steps[i] >= (1u << 24) || dst_steps[i] * (uint64_t)r->dst_h > 0xffffffffu)
-            return PANO_ERR_ARG;
-        ot.p[i] = dsts[i];
-        ot.step[i] = (uint32_t)dst_steps[i];
+        int e = output_entry(outs[i], r->dst_w, r->dst_h, ot, (int)i);
+        if (e) return e;
         for (int c = 0; c < r->rig->prep.n; c++, k++) {
             if (!srcs[k] || src_steps[k] < (size_t)r->src_w[c] * 4 || src_steps[k] % 4 ||
                 src_steps[k] * (uint64_t)r->src_h[c] > 0xffffffffu || src_steps[k] >= (1u << 24))
@@ -1563,14 +1678,13 @@ LutPass batch_lut_pass(const pano_batch* b, const SrcTable& st)
 
 extern "C" {
 
-int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
-                      const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
-                      hipStream_t stream)
+int pano_batch_render_to(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                         const uint8_t* const* luts, const pano_output* outs, hipStream_t stream)
 {
     SrcTable st, sr;
     OutTable ot;
     bool any = false;
-    int e = batch_prepare(b, srcs, src_steps, luts, dsts, dst_steps, stream, st, sr, ot, any);
+    int e = batch_prepare(b, srcs, src_steps, luts, outs, stream, st, sr, ot, any);
     if (e) return e;
     const LutPass lp = batch_lut_pass(b, st);
     e = run_sequence(b->seq, sr, ot, stream, b->timer.begin_frame(), any ? &lp : nullptr);
@@ -1581,9 +1695,8 @@ int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* s
     return PANO_OK;
 }
 
-int pano_batch_capture(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
-                       const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
-                       hipStream_t stream)
+int pano_batch_capture_to(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                          const uint8_t* const* luts, const pano_output* outs, hipStream_t stream)
 {
     if (!b || !stream) return PANO_ERR_ARG;   // the null stream cannot be captured
     b->drop_graph();
@@ -1591,7 +1704,7 @@ int pano_batch_capture(pano_batch* b, const uint8_t* const* srcs, const size_t* 
     OutTable ot;
     bool any = false;
     // uploads happen here, outside the capture: the graph holds kernel launches only
-    int e = batch_prepare(b, srcs, src_steps, luts, dsts, dst_steps, stream, st, sr, ot, any);
+    int e = batch_prepare(b, srcs, src_steps, luts, outs, stream, st, sr, ot, any);
     if (e) return e;
     const LutPass lp = batch_lut_pass(b, st);
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
@@ -1609,6 +1722,36 @@ int pano_batch_capture(pano_batch* b, const uint8_t* const* srcs, const size_t* 
     b->exec_luts.assign(b->renders.size(), 0);
     for (size_t i = 0; i < b->renders.size(); i++) b->exec_luts[i] = luts && luts[i];
     return PANO_OK;
+}
+
+namespace {
+// BGRx outputs of a batch as pano_output records (null when an argument is missing).
+bool bgrx_outputs(const pano_batch* b, uint8_t* const* dsts, const size_t* dst_steps,
+                  std::vector<pano_output>& outs)
+{
+    if (!b || !dsts || !dst_steps) return false;
+    outs.resize(b->renders.size());
+    for (size_t i = 0; i < outs.size(); i++) outs[i] = bgrx_output(dsts[i], dst_steps[i]);
+    return true;
+}
+}  // namespace
+
+int pano_batch_render(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                      const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                      hipStream_t stream)
+{
+    std::vector<pano_output> outs;
+    if (!bgrx_outputs(b, dsts, dst_steps, outs)) return PANO_ERR_ARG;
+    return pano_batch_render_to(b, srcs, src_steps, luts, outs.data(), stream);
+}
+
+int pano_batch_capture(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
+                       const uint8_t* const* luts, uint8_t* const* dsts, const size_t* dst_steps,
+                       hipStream_t stream)
+{
+    std::vector<pano_output> outs;
+    if (!bgrx_outputs(b, dsts, dst_steps, outs)) return PANO_ERR_ARG;
+    return pano_batch_capture_to(b, srcs, src_steps, luts, outs.data(), stream);
 }
 
 int pano_batch_replay(pano_batch* b, const uint8_t* const* luts, hipStream_t stream)

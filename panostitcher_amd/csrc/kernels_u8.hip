@@ -569,6 +569,26 @@ __global__ __launch_bounds__(kQT) void k_final_u8(const Seg* __restrict__ segs, 
             o[d] = (uint32_t)sat_u8(up[d].x + lap[d][0]) | ((uint32_t)sat_u8(up[d].y + lap[d][1]) << 8) |
                    ((uint32_t)sat_u8(up[d].z + lap[d][2]) << 16);
     }
+    const int fmt = outs.fmt[Sg.out];
+    if (fmt != kOutBGRX) {
+        // the encoder's planes straight from the quad (canvas sides are even, so
+        // every quad is whole): Y pairs, and U, V (interleaved for NV12)
+        if (!in[0]) return;
+        uint32_t yt, yb, u, v;
+        bt601_block(o, yt, yb, u, v);
+        uint8_t* yr = out + (__umul24((uint32_t)Y, ostep) + (uint32_t)X);
+        stg((uint16_t*)yr, (uint16_t)yt);
+        stg((uint16_t*)(yr + ostep), (uint16_t)yb);
+        const uint32_t cy = (uint32_t)Y >> 1;
+        if (fmt == kOutNV12) {
+            stg((uint16_t*)(outs.p2[Sg.out] + (__umul24(cy, outs.step2[Sg.out]) + (uint32_t)X)),
+                (uint16_t)(u | (v << 8)));
+        } else {
+            stg(outs.p2[Sg.out] + (__umul24(cy, outs.step2[Sg.out]) + ((uint32_t)X >> 1)), (uint8_t)u);
+            stg(outs.p3[Sg.out] + (__umul24(cy, outs.step3[Sg.out]) + ((uint32_t)X >> 1)), (uint8_t)v);
+        }
+        return;
+    }
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         uint32_t* row = (uint32_t*)(out + (__umul24((uint32_t)(Y + h), ostep) + 4u * (uint32_t)X));

@@ -163,9 +163,16 @@ struct SrcTable {
 };
 
 constexpr int kMaxOut = 32;     // output canvases addressable by one launch
+// Output formats of the level-0 kernel: the BGRx canvas, or the encoder's
+// planes (cvtBGR32ToNV12 / cvtBGR32ToYUV420P of that canvas) written directly.
+enum OutFormat : int { kOutBGRX = 0, kOutNV12 = 1, kOutYUV420P = 2 };
 struct OutTable {
-    uint8_t* p[kMaxOut];
+    uint8_t* p[kMaxOut];    // BGRx canvas, or the Y plane
     uint32_t step[kMaxOut];
+    uint8_t* p2[kMaxOut];   // NV12: interleaved UV; YUV420P: U
+    uint8_t* p3[kMaxOut];   // YUV420P: V
+    uint32_t step2[kMaxOut], step3[kMaxOut];
+    int fmt[kMaxOut];       // OutFormat (zero-initialised tables are BGRx)
 };
 
 }  // namespace pano

@@ -258,6 +258,59 @@ __device__ __forceinline__ void tap_offsets(uint32_t sstep, int sw, int sh, floa
     o[3] = o3;
 }
 
+// BT.601 studio swing in 20-bit fixed point: OpenCV's cvtColor table, which the
+// encoder hand-off uses (CudaAccel/ImageProc.cu:44-60).  Forward: Y of every
+// pixel of a 2x2 block, U and V from its left column (top and bottom averaged
+// with one more shift, ImageProc.cu:70-104).  Inverse: one (U, V) per block.
+namespace bt601 {
+constexpr int kRY = 269484, kGY = 528482, kBY = 102760;
+constexpr int kRU = -155188, kGU = -305135, kBU = 460324;
+constexpr int kGV = -385875, kBV = -74448;   // V's red coefficient is kBU
+constexpr int kY = 1220542, kUB = 2116026, kUG = -409993, kVG = -852492, kVR = 1673527;
+constexpr int kShift = 20, kHalf = 1 << 19;
+}  // namespace bt601
+
+__device__ __forceinline__ uint32_t bt601_y(uint32_t p)
+{
+    using namespace bt601;
+    const int v = kRY * (int)((p >> 16) & 255u) + kGY * (int)((p >> 8) & 255u) + kBY * (int)(p & 255u) + kHalf +
+                  (16 << kShift);
+    return (uint32_t)sat_u8(v >> kShift);
+}
+
+// q: the block's BGRx pixels (top-left, top-right, bottom-left, bottom-right).
+// yt / yb: the two Y bytes of the top / bottom row (left in the low byte).
+__device__ __forceinline__ void bt601_block(const uint32_t q[4], uint32_t& yt, uint32_t& yb, uint32_t& u,
+                                            uint32_t& v)
+{
+    using namespace bt601;
+    yt = bt601_y(q[0]) | (bt601_y(q[1]) << 8);
+    yb = bt601_y(q[2]) | (bt601_y(q[3]) << 8);
+    int us = 0, vs = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k += 2) {
+        const int b = (int)(q[k] & 255u), g = (int)((q[k] >> 8) & 255u), r = (int)((q[k] >> 16) & 255u);
+        us += kRU * r + kGU * g + kBU * b + kHalf + (128 << kShift);
+        vs += kBU * r + kGV * g + kBV * b + kHalf + (128 << kShift);
+    }
+    u = (uint32_t)sat_u8(us >> (kShift + 1));
+    v = (uint32_t)sat_u8(vs >> (kShift + 1));
+}
+
+// The inverse for one block: ys = the four Y bytes (top-left first), channel 3 = 0.
+__device__ __forceinline__ void bt601_to_bgr(const uint32_t ys[4], int uval, int vval, uint32_t q[4])
+{
+    using namespace bt601;
+    const int u = uval - 128, v = vval - 128;
+    const int ruv = kHalf + kVR * v, guv = kHalf + kVG * v + kUG * u, buv = kHalf + kUB * u;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const int y = max(0, (int)ys[k] - 16) * kY;
+        q[k] = (uint32_t)sat_u8((y + buv) >> kShift) | ((uint32_t)sat_u8((y + guv) >> kShift) << 8) |
+               ((uint32_t)sat_u8((y + ruv) >> kShift) << 16);
+    }
+}
+
 // The job a block of a batched launch runs.  Workgroups are dispatched to the
 // 8 XCDs round-robin (block b to XCD b mod 8), each XCD with its own L2.
 // sched 0: job = block.  sched 1: each XCD gets one contiguous run of jobs, so

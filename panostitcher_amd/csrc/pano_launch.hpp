@@ -86,6 +86,14 @@ hipError_t launch_blend_custom(int user_kind, const Plane* planes, const CanvasL
                                const int32_t* wsum, const SrcTable& src, const uint8_t* cover,
                                uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
 
+// Encoder / decoder hand-off (kernels_yuv.hip); w, h even; nv12: u is the
+// interleaved UV plane and v unused.
+hipError_t launch_bgr32_to_yuv(bool nv12, const uint8_t* src, size_t sstep, int w, int h, uint8_t* y,
+                               size_t ystep, uint8_t* u, size_t ustep, uint8_t* v, size_t vstep, hipStream_t s);
+hipError_t launch_yuv_to_bgr32(bool nv12, const uint8_t* y, size_t ystep, const uint8_t* u, size_t ustep,
+                               const uint8_t* v, size_t vstep, int w, int h, uint8_t* dst, size_t dstep,
+                               hipStream_t s);
+
 hipError_t launch_copy_plane(const void* src, size_t src_step, void* dst, size_t dst_step,
                              int row_bytes, int rows, hipStream_t s);
 
