@@ -156,6 +156,46 @@ int pano_blender32f_blend(pano_blender32f* b, int num, const void* const* images
                           int type, uint8_t* out, size_t out_step, hipStream_t stream);
 int pano_blender32f_num_levels(const pano_blender32f* b);
 
+/* ---- linear-blend mode (highQualityBlend = 0) ----------------------------
+ * Weights: getWeightsLinearBlend32F (as_int 0: float, calcWeights32F) or
+ * getWeightsLinearBlend (as_int 1: int32 in 2^16 units, calcWeights),
+ * Blend/LinearBlend.cpp:17-148: seams, a (2 radius + 1)^2 Gaussian blur of
+ * each seam mask (sigma radius / 3, OpenCV 3.0's 8-bit fixed-point path),
+ * ANDed with the mask, normalised per pixel.  masks: host; out: device planes
+ * of rows x cols elements (4 bytes each).  Synchronous. */
+int pano_linear_weights(int num, int rows, int cols, const uint8_t* const* masks, const size_t* mask_steps,
+                        int radius, int as_int, void* const* out, const size_t* out_steps);
+
+/* CudaTilingLinearBlend (CudaAccel/CudaInterface.h:103-115,
+ * Blend/CudaLinearBlend.cpp:15-101): prepare(masks, radius) (false on empty or
+ * mismatched masks or radius < 1), blend(images, blendImage): images device
+ * PANO_8UC4, out device PANO_8UC4 (all four channels: sum of weight x pixel,
+ * >> 16, saturated). */
+typedef struct pano_linear_blender pano_linear_blender;
+int pano_linear_blender_create(pano_linear_blender** out);
+void pano_linear_blender_destroy(pano_linear_blender* b);
+int pano_linear_blender_prepare(pano_linear_blender* b, int num, int rows, int cols, const uint8_t* const* masks,
+                                const size_t* mask_steps, int radius);
+int pano_linear_blender_blend(pano_linear_blender* b, int num, const uint8_t* const* images, const size_t* steps,
+                              uint8_t* out, size_t out_step, hipStream_t stream);
+
+/* The render's linear branch (prepare with highQualityBlend = 0, blendParam =
+ * radius; render: Task/RicohUtil.cpp:1269-1287, CPU twin :1543-1558): per
+ * frame, for each camera in order, the bilinear sample times its float weight
+ * is added in float, then the sum is rounded to 8 bits.  Maps and masks as
+ * for pano_render_prepare_maps (host, dst_w x dst_h); luts as for
+ * pano_render_render.  dst: device BGRx canvas (channel 3 written as 0). */
+typedef struct pano_linear_render pano_linear_render;
+int pano_linear_render_create(pano_linear_render** out);
+void pano_linear_render_destroy(pano_linear_render* r);
+int pano_linear_render_prepare_maps(pano_linear_render* r, int num, const int* src_w, const int* src_h, int dst_w,
+                                    int dst_h, const float* const* xmaps, const float* const* ymaps,
+                                    const uint8_t* const* masks, int radius);
+int pano_linear_render_render(pano_linear_render* r, int num, const uint8_t* const* srcs, const size_t* src_steps,
+                              const uint8_t* luts, uint8_t* dst, size_t dst_step, hipStream_t stream);
+/* The prepared float weights of camera cam (host dst_w x dst_h floats). */
+int pano_linear_render_weights(const pano_linear_render* r, int cam, float* out);
+
 /* ---- free functions ----------------------------------------------------- */
 
 /* cudaReprojectTo16S(src, dst, xmap, ymap, stream) (CudaInterface.h:150-151),

@@ -136,6 +136,19 @@ void orc_pyrup32f(const float* src, int sw, int sh, int cn, float* dst, int dw, 
 int orc_blend32f(int n, int rows, int cols, const uint8_t* masks, const float* images,
                  int max_levels, int min_length, uint8_t* out);
 
+/* Linear-blend mode (linear.c).  masks / weights: n planes rows x cols;
+ * as_int: int32 weights (calcWeights, 2^16 units) else float (calcWeights32F).
+ * srcs: BGR (3-byte) frames; out: dw x dh x 3.  images (blend32s): n planes of
+ * rows x cols x cn bytes. */
+void orc_gaussian_kernel_q8(int n, double sigma, int* K);
+void orc_gaussian_blur_u8(const uint8_t* src, int w, int h, int n, double sigma, uint8_t* dst);
+int orc_linear_weights(int n, int rows, int cols, const uint8_t* masks, int radius, int as_int, void* out);
+int orc_linear_render(int n, const uint8_t* const* srcs, const int* sw, const int* sh,
+                      const float* const* xmaps, const float* const* ymaps, const float* weights,
+                      int dw, int dh, const uint8_t* luts, uint8_t* out, int nthreads);
+int orc_linear_blend32s(int n, int rows, int cols, int cn, const uint8_t* images, const int32_t* weights,
+                        uint8_t* out);
+
 /* Encoder / decoder hand-off (yuv.c): CudaAccel/ImageProc.cu.  nv12: u is the
  * interleaved UV plane (w bytes per row, v unused); else u and v are w/2 x h/2.
  * y is w x h; w and h even.  Return 0 on bad sizes. */

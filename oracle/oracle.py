@@ -49,6 +49,16 @@ def lib():
         L.orc_blender_prepare.restype = i
         L.orc_blender_blend.argtypes = [vp, vp, vp, i]
         L.orc_blender_blend.restype = i
+        L.orc_gaussian_kernel_q8.argtypes = [i, C.c_double, vp]
+        L.orc_gaussian_kernel_q8.restype = None
+        L.orc_gaussian_blur_u8.argtypes = [vp, i, i, i, C.c_double, vp]
+        L.orc_gaussian_blur_u8.restype = None
+        L.orc_linear_weights.argtypes = [i, i, i, vp, i, i, vp]
+        L.orc_linear_weights.restype = i
+        L.orc_linear_render.argtypes = [i, vp, vp, vp, vp, vp, vp, i, i, vp, vp, i]
+        L.orc_linear_render.restype = i
+        L.orc_linear_blend32s.argtypes = [i, i, i, i, vp, vp, vp]
+        L.orc_linear_blend32s.restype = i
         L.orc_bgr32_to_yuv.argtypes = [vp, C.c_size_t, i, i, i, vp, vp, vp]
         L.orc_bgr32_to_yuv.restype = i
         L.orc_yuv_to_bgr32.argtypes = [vp, vp, vp, i, i, i, vp, C.c_size_t]
@@ -349,6 +359,60 @@ def blend32f(masks: np.ndarray, images: np.ndarray, max_levels: int, min_length:
     out = np.zeros((h, w, 4), np.uint8)
     L = lib().orc_blend32f(n, h, w, _p(masks), _p(images), max_levels, min_length, _p(out))
     return out, L
+
+
+def gaussian_kernel_q8(n: int, sigma: float) -> np.ndarray:
+    """getGaussianKernel(n, sigma, CV_32F) as 8-bit fixed-point taps."""
+    K = np.zeros(n, np.int32)
+    lib().orc_gaussian_kernel_q8(n, sigma, _p(K))
+    return K
+
+
+def gaussian_blur_u8(src: np.ndarray, n: int, sigma: float) -> np.ndarray:
+    src = np.ascontiguousarray(src, np.uint8)
+    h, w = src.shape
+    out = np.zeros_like(src)
+    lib().orc_gaussian_blur_u8(_p(src), w, h, n, sigma, _p(out))
+    return out
+
+
+def linear_weights(masks: np.ndarray, radius: int, as_int: bool = False) -> np.ndarray:
+    """getWeightsLinearBlend32F (float) / getWeightsLinearBlend (int32, 2^16 units)."""
+    masks = np.ascontiguousarray(masks, np.uint8)
+    n, h, w = masks.shape
+    out = np.zeros((n, h, w), np.int32 if as_int else np.float32)
+    assert lib().orc_linear_weights(n, h, w, _p(masks), radius, int(as_int), _p(out))
+    return out
+
+
+def linear_render(srcs, xmaps, ymaps, weights, luts=None, nthreads=8) -> np.ndarray:
+    """CPUPanoramaRender::render, linear branch, on BGR frames -> (h, w, 3) u8."""
+    n = len(srcs)
+    srcs = [np.ascontiguousarray(s[..., :3]) for s in srcs]
+    xmaps = [np.ascontiguousarray(x, np.float32) for x in xmaps]
+    ymaps = [np.ascontiguousarray(y, np.float32) for y in ymaps]
+    weights = np.ascontiguousarray(weights, np.float32)
+    dh, dw = xmaps[0].shape
+    sw = (C.c_int * n)(*[s.shape[1] for s in srcs])
+    sh = (C.c_int * n)(*[s.shape[0] for s in srcs])
+    lp = None
+    if luts is not None:
+        luts = np.ascontiguousarray(luts, np.uint8)
+        lp = _p(luts)
+    out = np.zeros((dh, dw, 3), np.uint8)
+    assert lib().orc_linear_render(n, _ptrs(srcs), sw, sh, _ptrs(xmaps), _ptrs(ymaps), _p(weights), dw, dh,
+                                   lp, _p(out), nthreads)
+    return out
+
+
+def linear_blend32s(images: np.ndarray, weights: np.ndarray) -> np.ndarray:
+    """TilingLinearBlend::blend / CudaTilingLinearBlend::blend: images (n, h, w, cn) u8."""
+    images = np.ascontiguousarray(images, np.uint8)
+    weights = np.ascontiguousarray(weights, np.int32)
+    n, h, w, cn = images.shape
+    out = np.zeros((h, w, cn), np.uint8)
+    assert lib().orc_linear_blend32s(n, h, w, cn, _p(images), _p(weights), _p(out))
+    return out
 
 
 def bgr32_to_yuv420p(bgrx: np.ndarray):

@@ -78,6 +78,16 @@ def lib():
             "pano_blender_num_levels": ([vp], i),
             "pano_blender_mem_bytes": ([vp], i64),
             "pano_reproject": ([vp, sz, i, i, vp, vp, sz, vp, sz, i, i, vp], i),
+            "pano_linear_weights": ([i, i, i, pp, psz, i, i, pp, psz], i),
+            "pano_linear_blender_create": ([pp], i),
+            "pano_linear_blender_destroy": ([vp], None),
+            "pano_linear_blender_prepare": ([vp, i, i, i, pp, psz, i], i),
+            "pano_linear_blender_blend": ([vp, i, pp, psz, vp, sz, vp], i),
+            "pano_linear_render_create": ([pp], i),
+            "pano_linear_render_destroy": ([vp], None),
+            "pano_linear_render_prepare_maps": ([vp, i, vp, vp, i, i, pp, pp, pp, i], i),
+            "pano_linear_render_render": ([vp, i, pp, psz, vp, vp, sz, vp], i),
+            "pano_linear_render_weights": ([vp, i, vp], i),
             "pano_blender32f_create": ([pp], i),
             "pano_blender32f_destroy": ([vp], None),
             "pano_blender32f_prepare": ([vp, i, i, i, pp, psz, i, i], i),
@@ -164,7 +174,10 @@ def exported_symbols():
         "pano_blender32f_blend", "pano_blender32f_num_levels", "pano_reproject",
         "pano_blender_blend_custom", "pano_render_render_to", "pano_batch_render_to",
         "pano_batch_capture_to", "pano_bgr32_to_yuv420p", "pano_bgr32_to_nv12",
-        "pano_yuv420p_to_bgr32", "pano_nv12_to_bgr32",
+        "pano_yuv420p_to_bgr32", "pano_nv12_to_bgr32", "pano_linear_weights",
+        "pano_linear_blender_create", "pano_linear_blender_destroy", "pano_linear_blender_prepare",
+        "pano_linear_blender_blend", "pano_linear_render_create", "pano_linear_render_destroy",
+        "pano_linear_render_prepare_maps", "pano_linear_render_render", "pano_linear_render_weights",
     ]
 
 
@@ -383,6 +396,109 @@ class CudaTilingMultibandBlendFast:
     @staticmethod
     def estimate_memory_size(width: int, height: int, num_images: int) -> int:
         return lib().pano_estimate_mem_bytes(width, height, num_images)
+
+
+def linear_weights(masks, radius: int, as_int: bool = False):
+    """getWeightsLinearBlend32F / getWeightsLinearBlend on the device: masks host
+    (n, h, w) uint8 -> (n, h, w) float32 (int32 in 2^16 units) cuda tensor."""
+    import torch
+
+    masks = np.ascontiguousarray(masks, np.uint8)
+    n, h, w = masks.shape
+    out = torch.empty((n, h, w), dtype=torch.int32 if as_int else torch.float32, device="cuda")
+    _check(lib().pano_linear_weights(n, h, w, _ptrs([m.ctypes.data for m in masks]), _sizes([w] * n),
+                                     int(radius), int(as_int), _ptrs([out[i].data_ptr() for i in range(n)]),
+                                     _sizes([w * 4] * n)), "linear weights")
+    return out
+
+
+class CudaTilingLinearBlend:
+    """CudaTilingLinearBlend (CudaAccel/CudaInterface.h:103-115)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().pano_linear_blender_create(C.byref(h)), "create linear blender")
+        self._h = h
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pano_linear_blender_destroy(h)
+            self._h = None
+
+    def prepare(self, masks, radius: int) -> bool:
+        if len(masks) == 0:
+            return False
+        arrs = [np.ascontiguousarray(m, np.uint8) for m in masks]
+        h, w = arrs[0].shape
+        if any(a.shape != (h, w) for a in arrs):
+            return False
+        return lib().pano_linear_blender_prepare(self._h, len(arrs), h, w, _ptrs([a.ctypes.data for a in arrs]),
+                                                 _sizes([w] * len(arrs)), int(radius)) == PANO_OK
+
+    def blend(self, images, out, stream=None) -> bool:
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in images])
+        op, ostep = _dev_ptr_step(out)
+        st = lib().pano_linear_blender_blend(self._h, len(images), _ptrs(ptrs), _sizes(steps), C.c_void_p(op),
+                                             ostep, _stream(stream))
+        if st == -2:
+            return False
+        _check(st, "linear blend")
+        return True
+
+
+class LinearRender:
+    """The render's linear-blend mode (highQualityBlend = 0, blendParam = radius)."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        _check(lib().pano_linear_render_create(C.byref(h)), "create linear render")
+        self._h = h
+        self.n = 0
+        self.dst = (0, 0)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h and _lib is not None:
+            _lib.pano_linear_render_destroy(h)
+            self._h = None
+
+    def prepare_maps(self, src_sizes, dst_size, xmaps, ymaps, masks, radius: int) -> bool:
+        n = len(src_sizes)
+        dw, dh = dst_size
+        xs = [np.ascontiguousarray(x, np.float32) for x in xmaps]
+        ys = [np.ascontiguousarray(y, np.float32) for y in ymaps]
+        ms = [np.ascontiguousarray(m, np.uint8) for m in masks]
+        sw = (C.c_int * n)(*[s[0] for s in src_sizes])
+        sh = (C.c_int * n)(*[s[1] for s in src_sizes])
+        st = lib().pano_linear_render_prepare_maps(self._h, n, sw, sh, dw, dh,
+                                                   _ptrs([a.ctypes.data for a in xs]),
+                                                   _ptrs([a.ctypes.data for a in ys]),
+                                                   _ptrs([a.ctypes.data for a in ms]), int(radius))
+        if st == PANO_OK:
+            self.n, self.dst = n, (dw, dh)
+        return st == PANO_OK
+
+    def render(self, srcs, out, luts=None, stream=None) -> bool:
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in srcs])
+        op, ostep = _dev_ptr_step(out)
+        lp = None
+        if luts is not None:
+            luts = np.ascontiguousarray(luts, np.uint8)
+            assert luts.shape == (len(srcs), 3, 256)
+            lp = C.c_void_p(luts.ctypes.data)
+        st = lib().pano_linear_render_render(self._h, len(srcs), _ptrs(ptrs), _sizes(steps), lp, C.c_void_p(op),
+                                             ostep, _stream(stream))
+        if st == -2:
+            return False
+        _check(st, "linear render")
+        return True
+
+    def weights(self, cam: int) -> np.ndarray:
+        dw, dh = self.dst
+        out = np.zeros((dh, dw), np.float32)
+        _check(lib().pano_linear_render_weights(self._h, cam, C.c_void_p(out.ctypes.data)), "weights")
+        return out
 
 
 class CudaTilingMultibandBlendFast32F:

@@ -1,0 +1,189 @@
+// kernels_linear.hip -- the linear-blend mode (highQualityBlend = 0) on gfx950.
+//
+//   k_blur_rows / k_blur_cols   cv::GaussianBlur of an 8-bit seam mask on
+//       OpenCV 3.0's 8-bit fixed-point path: integer taps K (256 x the float
+//       Gaussian, rounded), rows to int32, then columns, (s + 2^15) >> 16
+//       saturated; borders REFLECT_101 (Blend/LinearBlend.cpp:134-148).  The
+//       result is ANDed bitwise with the camera's mask.
+//   k_linear_weights            calcWeights32F (:76-132) / calcWeights (:17-57)
+//       per canvas pixel over the cameras' blurred seams.
+//   k_linear_render             per canvas pixel, the cameras listed for its
+//       tile in order: bilinear sample (exact, as the remap path) times the
+//       weight, summed in float without contraction
+//       (reprojectWeightedAccumulateTo32F, Warp/ZReproject.cpp:606-691), then
+//       rounded half to even and saturated (convertTo(CV_8U)).
+//   k_linear_blend32s           CudaTilingLinearBlend::blend
+//       (CudaAccel/CudaFeatherBlend.cu:8-29): int32 += weight x pixel over the
+//       cameras, >> 16, saturated, all four channels.
+// Weight and map planes are full-canvas per camera, camera after camera.
+#include <hip/hip_runtime.h>
+
+#include "pano_dev.hpp"
+#include "pano_kernel_common.hpp"
+#include "pano_launch.hpp"
+
+namespace pano {
+
+__global__ __launch_bounds__(256) void k_blur_rows(const uint8_t* __restrict__ src, int pitch, int w, int h,
+                                                   const int* __restrict__ K, int n, int k0, int k1,
+                                                   int32_t* __restrict__ dst)
+{
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63), Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (X >= w || Y >= h) return;
+    const uint8_t* row = src + (size_t)Y * pitch;
+    const int a = n / 2;
+    int s = 0;
+    for (int k = k0; k < k1; k++) s += ldg(K + k) * (int)ldg(row + breflect(X + k - a, w));
+    dst[(size_t)Y * pitch + X] = s;
+}
+
+__global__ __launch_bounds__(256) void k_blur_cols(const int32_t* __restrict__ rows, int pitch, int w, int h,
+                                                   const int* __restrict__ K, int n, int k0, int k1,
+                                                   const uint8_t* __restrict__ mask, int mpitch,
+                                                   uint8_t* __restrict__ dst)
+{
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63), Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (X >= w || Y >= h) return;
+    const int a = n / 2;
+    int s = 0;
+    for (int k = k0; k < k1; k++) s += ldg(K + k) * ldg(rows + (size_t)breflect(Y + k - a, h) * pitch + X);
+    const int v = sat_u8((s + (1 << 15)) >> 16);
+    dst[(size_t)Y * pitch + X] = (uint8_t)(v & ldg(mask + (size_t)Y * mpitch + X));
+}
+
+// dist: n planes (pitch bytes per row, cam_stride bytes apart); out: n planes
+// of float (AS_INT: int32) with the same pitch in elements.
+template <bool AS_INT>
+__global__ __launch_bounds__(256) void k_linear_weights(const uint8_t* __restrict__ dist, int pitch,
+                                                        size_t cam_stride, int n, int w, int h,
+                                                        void* __restrict__ out)
+{
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63), Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (X >= w || Y >= h) return;
+    const size_t q = (size_t)Y * pitch + X;
+    float sum = 0.f;
+    int nz = 0, last = 0;
+    for (int k = 0; k < n; k++) {
+        const int d = ldg(dist + k * cam_stride + q);
+        sum = __fadd_rn(sum, (float)d);
+        if (d) {
+            nz++;
+            last = k;
+        }
+    }
+    const float inv = __builtin_fabsf(sum) <= __FLT_MIN__ ? 0.f : __fdiv_rn(1.0f, sum);
+    for (int k = 0; k < n; k++) {
+        const float d = (float)ldg(dist + k * cam_stride + q);
+        if (AS_INT) {
+            // (int)(d * inv * 2^16 + 1), truncated
+            const float t = __fadd_rn(__fmul_rn(__fmul_rn(d, inv), 65536.0f), 1.0f);
+            ((int32_t*)out)[k * cam_stride + q] = (int32_t)t;
+        } else {
+            float wv = 0.f;
+            if (nz > 1) wv = __fmul_rn(d, inv);
+            else if (nz == 1 && k == last) wv = 1.0f;
+            ((float*)out)[k * cam_stride + q] = wv;
+        }
+    }
+}
+
+// One canvas tile of the linear render: cameras camlist[J.off .. J.off + J.cnt).
+__global__ __launch_bounds__(256) void k_linear_render(const LinTile* __restrict__ tiles,
+                                                       const int* __restrict__ camlist,
+                                                       const LinCam* __restrict__ cams, SrcTable src,
+                                                       uint8_t* __restrict__ out, uint32_t ostep, int w, int h)
+{
+    const LinTile J = ldc(tiles + blockIdx.x);
+    const int X = J.X + (threadIdx.x & 63), Y = J.Y + (threadIdx.x >> 6);
+    if (X >= w || Y >= h) return;
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    for (int k = 0; k < J.cnt; k++) {
+        const LinCam C = ldc(cams + ldc(camlist + J.off + k));
+        const size_t q = (size_t)Y * C.pitch + X;
+        const float wv = ldg(C.w + q);
+        if (wv == 0.f) continue;   // adds nothing
+        const float2 m = ldg_f2(C.map + q);
+        if (!(m.x >= 0.f && m.y >= 0.f && m.x < (float)C.sw && m.y < (float)C.sh)) continue;
+        const uint8_t* sp = src.p[C.src];
+        uint32_t o[4], p[4];
+        tap_offsets(src.step[C.src], C.sw, C.sh, m.x, m.y, o);
+#pragma unroll
+        for (int t = 0; t < 4; t++) p[t] = ldg((const uint32_t*)(sp + o[t]));
+        bool tie = false;
+        uint32_t v = bilinear_fast(__builtin_amdgcn_fractf(m.x), __builtin_amdgcn_fractf(m.y), p, tie);
+        if (tie) v = bilinear_combine((double)m.x, (double)m.y, p, nullptr);
+        a0 = __fadd_rn(a0, __fmul_rn((float)(v & 255u), wv));
+        a1 = __fadd_rn(a1, __fmul_rn((float)((v >> 8) & 255u), wv));
+        a2 = __fadd_rn(a2, __fmul_rn((float)((v >> 16) & 255u), wv));
+    }
+    uint32_t o = __builtin_amdgcn_cvt_pk_u8_f32(a0, 0u, 0u);
+    o = __builtin_amdgcn_cvt_pk_u8_f32(a1, 1u, o);
+    o = __builtin_amdgcn_cvt_pk_u8_f32(a2, 2u, o);
+    stg((uint32_t*)(out + (size_t)Y * ostep) + X, o);
+}
+
+__global__ __launch_bounds__(256) void k_linear_blend32s(SrcTable images, int n, const int32_t* __restrict__ wts,
+                                                         int pitch, size_t cam_stride, uint8_t* __restrict__ out,
+                                                         uint32_t ostep, int w, int h)
+{
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63), Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (X >= w || Y >= h) return;
+    const size_t q = (size_t)Y * pitch + X;
+    // weights are 1 .. 2^16 + 1 (calcWeights): 24-bit products suffice
+    uint32_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    for (int i = 0; i < n; i++) {
+        const uint32_t wv = (uint32_t)ldg(wts + i * cam_stride + q);
+        const uint32_t p = ldg((const uint32_t*)(images.p[i] + (size_t)Y * images.step[i]) + X);
+        s0 += __umul24(wv, p & 255u);
+        s1 += __umul24(wv, (p >> 8) & 255u);
+        s2 += __umul24(wv, (p >> 16) & 255u);
+        s3 += __umul24(wv, p >> 24);
+    }
+    const uint32_t o = (uint32_t)sat_u8((int)(s0 >> 16)) | ((uint32_t)sat_u8((int)(s1 >> 16)) << 8) |
+                       ((uint32_t)sat_u8((int)(s2 >> 16)) << 16) | ((uint32_t)sat_u8((int)(s3 >> 16)) << 24);
+    stg((uint32_t*)(out + (size_t)Y * ostep) + X, o);
+}
+
+namespace {
+dim3 grid_px(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }
+}  // namespace
+
+hipError_t launch_blur_u8(const uint8_t* src, int pitch, int w, int h, const int* K, int n, int k0, int k1,
+                          int32_t* rows, const uint8_t* mask, int mpitch, uint8_t* dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_blur_rows, grid_px(w, h), dim3(256), 0, s, src, pitch, w, h, K, n, k0, k1, rows);
+    hipLaunchKernelGGL(k_blur_cols, grid_px(w, h), dim3(256), 0, s, rows, pitch, w, h, K, n, k0, k1, mask,
+                       mpitch, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_linear_weights(bool as_int, const uint8_t* dist, int pitch, size_t cam_stride, int n, int w,
+                                 int h, void* out, hipStream_t s)
+{
+    if (as_int)
+        hipLaunchKernelGGL(k_linear_weights<true>, grid_px(w, h), dim3(256), 0, s, dist, pitch, cam_stride, n, w,
+                           h, out);
+    else
+        hipLaunchKernelGGL(k_linear_weights<false>, grid_px(w, h), dim3(256), 0, s, dist, pitch, cam_stride, n,
+                           w, h, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_linear_render(const LinTile* tiles, int ntiles, const int* camlist, const LinCam* cams,
+                                const SrcTable& src, uint8_t* out, size_t ostep, int w, int h, hipStream_t s)
+{
+    if (ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_linear_render, dim3(ntiles), dim3(256), 0, s, tiles, camlist, cams, src, out,
+                       (uint32_t)ostep, w, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_linear_blend32s(const SrcTable& images, int n, const int32_t* wts, int pitch, size_t cam_stride,
+                                  uint8_t* out, size_t ostep, int w, int h, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_linear_blend32s, grid_px(w, h), dim3(256), 0, s, images, n, wts, pitch, cam_stride, out,
+                       (uint32_t)ostep, w, h);
+    return hipGetLastError();
+}
+
+}  // namespace pano

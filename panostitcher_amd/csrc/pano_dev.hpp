@@ -162,6 +162,17 @@ struct SrcTable {
     uint32_t step[kMaxSrc];
 };
 
+// Linear-blend render: a camera's full-canvas map and weight planes (pitch in
+// elements) and its frame (SrcTable index, size); a canvas tile of 64 x 4
+// pixels with its cameras camlist[off .. off + cnt).
+struct LinCam {
+    const float2* map;
+    const float* w;
+    int pitch, src, sw, sh;
+};
+struct LinTile { int X, Y, off, cnt; };
+constexpr int kLinTW = 64, kLinTH = 4;
+
 constexpr int kMaxOut = 32;     // output canvases addressable by one launch
 // Output formats of the level-0 kernel: the BGRx canvas, or the encoder's
 // planes (cvtBGR32ToNV12 / cvtBGR32ToYUV420P of that canvas) written directly.

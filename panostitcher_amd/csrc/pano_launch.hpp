@@ -94,6 +94,17 @@ hipError_t launch_yuv_to_bgr32(bool nv12, const uint8_t* y, size_t ystep, const 
                                const uint8_t* v, size_t vstep, int w, int h, uint8_t* dst, size_t dstep,
                                hipStream_t s);
 
+// Linear-blend mode (kernels_linear.hip).  Blur: src / dst / rows share the
+// row pitch (elements); taps K[k0 .. k1) are the non-zero ones of n.
+hipError_t launch_blur_u8(const uint8_t* src, int pitch, int w, int h, const int* K, int n, int k0, int k1,
+                          int32_t* rows, const uint8_t* mask, int mpitch, uint8_t* dst, hipStream_t s);
+hipError_t launch_linear_weights(bool as_int, const uint8_t* dist, int pitch, size_t cam_stride, int n, int w,
+                                 int h, void* out, hipStream_t s);
+hipError_t launch_linear_render(const LinTile* tiles, int ntiles, const int* camlist, const LinCam* cams,
+                                const SrcTable& src, uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
+hipError_t launch_linear_blend32s(const SrcTable& images, int n, const int32_t* wts, int pitch, size_t cam_stride,
+                                  uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
+
 hipError_t launch_copy_plane(const void* src, size_t src_step, void* dst, size_t dst_step,
                              int row_bytes, int rows, hipStream_t s);
 

@@ -1,0 +1,87 @@
+"""GPU parity of the linear-blend mode against the oracle: the weights
+(float and int32), CudaTilingLinearBlend, and the render's linear branch.
+The blur and weights are integer / correctly rounded float operations in the
+reference's order, and the weighted sum is accumulated in float in camera
+order without contraction: every result is expected bit-exact."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import panostitcher_amd as P  # noqa: E402
+from oracle import oracle as O  # noqa: E402
+from panostitcher_amd import synth  # noqa: E402
+from tests.gpu_util import assert_parity, from_dev, note, to_dev  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    P.lib()
+    yield
+
+
+def band_masks(rng, n, h, w):
+    m = np.zeros((n, h, w), np.uint8)
+    for i in range(n):
+        ov = int(rng.integers(2, 9))
+        cols = np.arange(i * w // n - ov, (i + 1) * w // n + ov) % w
+        y0, y1 = int(rng.integers(0, h // 4)), int(rng.integers(3 * h // 4, h + 1))
+        m[i][y0:y1][:, cols] = 255
+    return m
+
+
+@pytest.mark.parametrize("seed,n,h,w,r", [(0, 3, 40, 96, 4), (1, 2, 33, 70, 1), (2, 4, 64, 128, 12),
+                                          (3, 3, 24, 50, 0)])
+def test_weights_match_oracle(seed, n, h, w, r):
+    rng = np.random.default_rng(seed)
+    masks = band_masks(rng, n, h, w)
+    note(f"linear weights seed={seed} n={n} {w}x{h} r={r}")
+    for as_int in (False, True):
+        got = from_dev(P.linear_weights(masks, r, as_int))
+        ref = O.linear_weights(masks, r, as_int)
+        np.testing.assert_array_equal(got, ref)
+
+
+def test_linear_blender_matches_oracle():
+    rng = np.random.default_rng(7)
+    n, h, w, r = 3, 48, 100, 5
+    masks = band_masks(rng, n, h, w)
+    imgs = rng.integers(0, 256, size=(n, h, w, 4)).astype(np.uint8)
+    ref = O.linear_blend32s(imgs, O.linear_weights(masks, r, as_int=True))
+    note("linear blender 100x48")
+    b = P.CudaTilingLinearBlend()
+    assert not b.prepare(list(masks), 0)   # radius < 1 is refused, as in the reference
+    assert b.prepare(list(masks), r)
+    out = torch.zeros((h, w, 4), dtype=torch.uint8, device="cuda")
+    assert b.blend([to_dev(imgs[i]) for i in range(n)], out)
+    np.testing.assert_array_equal(from_dev(out), ref)
+
+
+@pytest.mark.parametrize("name,scale,r", [("fisheye2", 0.25, 8), ("rect6", 0.25, 6), ("mixed8", 0.125, 10)])
+def test_linear_render_matches_oracle(name, scale, r):
+    wl = synth.workload(name, scale)
+    xs, ys, ms = synth.maps_and_masks(wl)
+    srcs = synth.frames(wl)
+    luts = synth.luts(wl)
+    w = O.linear_weights(np.stack(ms), r)
+    ref = O.linear_render(srcs, xs, ys, w, luts)
+    note(f"linear render {name}@{scale} {wl.dst_w}x{wl.dst_h} n={wl.n} r={r} luts={luts is not None}")
+    lr = P.LinearRender()
+    assert lr.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h), xs, ys, ms, r)
+    for i in range(wl.n):
+        np.testing.assert_array_equal(lr.weights(i), w[i])
+    dsrc = [to_dev(synth.bgrx(s)) for s in srcs]
+    out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    for _ in range(2):
+        assert lr.render(dsrc, out, luts)
+        got = from_dev(out)
+        assert_parity(got[..., :3], ref, exact=True, label=f"linear {name}")
+        assert not got[..., 3].any()
+    # without LUTs after a frame with them
+    ref0 = O.linear_render(srcs, xs, ys, w, None)
+    assert lr.render(dsrc, out, None)
+    assert_parity(from_dev(out)[..., :3], ref0, exact=True, label=f"linear {name} no luts")
