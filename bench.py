@@ -24,6 +24,8 @@ Other configurations of the same path (one JSON line each, same fields):
       submits 30 frames/s; then the same 16 streams kept saturated
   c5  4 fisheyes 3840x2160 200 deg -> 7680x3840, L=10, gains re-estimated from
       overlap means every frame under a seeded brightness random walk
+  lin the mixed8 rig in the render's linear-blend mode (highQualityBlend = 0):
+      Gaussian-blurred seam weights, one weighted-remap launch per frame
 
 The roofline object gives the dominant stage's algorithmic bytes (the 16-bit
 stage model below) over its mean device time, measured with HIP events on the
@@ -438,6 +440,80 @@ def run_single(args, name, label, with_plain):
     return line
 
 
+def run_linear(args):
+    """lin: the render's linear-blend mode (highQualityBlend = 0) on the mixed8 rig:
+    Gaussian-blurred seam weights at prepare, then one weighted-remap launch per
+    frame (pano_linear_render)."""
+    import torch
+
+    import panostitcher_amd as P
+
+    wl, xs, ys, ms, srcs, luts = setup(args.workload or "mixed8", args.scale)
+    radius = args.linear_radius
+    lr = P.LinearRender()
+    t0 = time.perf_counter()
+    assert lr.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h), xs, ys, ms, radius)
+    prep_s = time.perf_counter() - t0
+    log(f"linear prepare (radius {radius}) in {prep_s:.2f}s")
+    dsrc = [torch.from_numpy(s).cuda() for s in srcs]
+    out = torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    stream = torch.cuda.current_stream()
+    dt = timed(lambda: lr.render(dsrc, out, luts, stream), args.steps, args.warmup)
+    # device time of the frames (one launch each without LUTs) with events on the stream
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(args.steps):
+        lr.render(dsrc, out, luts, stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    kms = ev0.elapsed_time(ev1) / args.steps
+    # algorithmic bytes per frame: a weight per listed (camera, pixel), a map point and
+    # the source pixel's 3 bytes per sampled one, the 4-byte output per canvas pixel
+    listed = sampled = 0
+    for i in range(wl.n):
+        w = lr.weights(i)
+        sampled += int(((w > 0) & (xs[i] >= 0) & (ys[i] >= 0) & (xs[i] < wl.cameras[i].width)
+                        & (ys[i] < wl.cameras[i].height)).sum())
+        m = ms[i] != 0
+        th, tw = (wl.dst_h + 3) // 4, (wl.dst_w + 63) // 64
+        pad = np.zeros((th * 4, tw * 64), bool)
+        pad[:wl.dst_h, :wl.dst_w] = m
+        listed += int(pad.reshape(th, 4, tw, 64).any(axis=(1, 3)).sum()) * 256
+    src_px = sum(c.width * c.height for c in wl.cameras)
+    bytes_frame = 4 * listed + 8 * sampled + 3 * src_px + 4 * wl.out_pixels
+    achieved = bytes_frame / (kms * 1e-3) / 1e9
+    value = wl.out_pixels * args.steps / dt / 1e6
+    line = base_line(f"stitched output Mpix/s (linear blend, {wl.name} -> {wl.dst_w}x{wl.dst_h})", value,
+                     args.steps, args.warmup, dt * 1e3 / args.steps,
+                     {"workload": f"{wl.name} -> {wl.dst_w}x{wl.dst_h}, linear blend radius {radius}, "
+                                  f"{wl.n} cams, one frame per step", "streams": 1,
+                      "parallelism": "single GPU"})
+    line["prepare_s"] = round(prep_s, 3)
+    line["roofline"] = {"bound": "hbm", "kernel": "linear_render", "achieved": round(achieved, 1),
+                        "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": (read_traffic(f"{wl.name}_linear") or {}).get("linear_render"),
+                        "algorithmic_bytes": int(bytes_frame),
+                        "model": "4 B weight per listed camera pixel + 8 B map and 3 B source per sample "
+                                 "+ 3 B per source pixel + 4 B output per canvas pixel",
+                        "kernel_ms": round(kms, 4)}
+    line["cpu_baseline"] = None
+    if not args.no_cpu and args.cpu_frames > 0:
+        from oracle import oracle as O
+
+        threads = min(16, os.cpu_count() or 1)
+        wts = np.stack([lr.weights(i) for i in range(wl.n)])
+        bgr = [s[..., :3].copy() for s in srcs]
+        t0 = time.perf_counter()
+        for _ in range(args.cpu_frames):
+            O.linear_render(bgr, xs, ys, wts, luts, nthreads=threads)
+        cdt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": round(wl.out_pixels * args.cpu_frames / cdt / 1e6, 3), "unit": "Mpix/s",
+                                "cores": threads, "kind": "port",
+                                "sample": f"{args.cpu_frames} frame(s), oracle linear-branch restatement "
+                                          f"(reprojection row-parallel, {threads} threads), {cdt:.1f} s"}
+    return line
+
+
 def run_autogain(args):
     """c5: fisheye4, gains re-estimated from overlap means every frame."""
     import torch
@@ -653,7 +729,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "c5", "lin"])
+    ap.add_argument("--linear-radius", type=int, default=96)
     ap.add_argument("--serve-seconds", type=float, default=6.0)
     ap.add_argument("--hbm-budget-gb", type=float, default=10.0)
     ap.add_argument("--workload", default=None)
@@ -696,6 +773,8 @@ def main():
         line = run_single(args, "rect6", "6 rect 1920x1080 -> 3840x1920, gains", True)
     elif args.config == "c4":
         line = run_served(args)
+    elif args.config == "lin":
+        line = run_linear(args)
     else:
         line = run_autogain(args)
     print(json.dumps(line), flush=True)
