@@ -260,8 +260,9 @@ def run_batched(args):
     for _ in range(S):
         renders.append(new_render(wl, xs, ys, ms))
     r0 = renders[0]
+    prep_each = (time.perf_counter() - t0) / S
     log(f"prepared {S} renders in {time.perf_counter() - t0:.1f}s, levels {r0.num_levels}, "
-        f"{r0.mem_bytes() / 1e9:.2f} GB each")
+        f"{r0.mem_bytes() / 1e9:.2f} GB each; last prepare by phase (ms) {renders[-1].prepare_ms()}")
     dsrc = [torch.from_numpy(s).cuda() for s in srcs]
     stream_src = [dsrc] + [[t.clone() for t in dsrc] for _ in range(S - 1)]
     outs = [torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda") for _ in range(S)]
@@ -372,6 +373,7 @@ def run_batched(args):
                           "p99": round(float(np.percentile(lat, 99)), 4),
                           "streams": S,
                           "note": "device time of a step; each stream's frame completes with it"}
+    line["prepare"] = {"s_per_rig": round(prep_each, 3), "last_ms_by_phase": renders[-1].prepare_ms()}
     line["single_stream"] = single
     line["batched_eager"] = eager
     line["batched_graph"] = graph

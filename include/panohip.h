@@ -207,6 +207,13 @@ int pano_reproject_to16s(const uint8_t* src, size_t src_step, int src_w, int src
                          const float* xmap, const float* ymap, size_t map_step, int16_t* dst,
                          size_t dst_step, int dst_w, int dst_h, hipStream_t stream);
 
+/* getNonIntersectingMasks (Blend/Mask.cpp:376-397, over cv::distanceTransform
+ * L1 3x3) computed on the device: host masks in, host unique masks (0 / 255,
+ * camera 0 keeps its mask values) out.  Synchronous.  The blender and render
+ * prepares use the same device path. */
+int pano_nonintersecting_masks(int num, int rows, int cols, const uint8_t* const* masks, const size_t* mask_steps,
+                               uint8_t* const* out, const size_t* out_steps);
+
 /* cudaReproject(src, dst, xmap, ymap, stream) (CudaInterface.h:147-148), with
  * the same bilinear arithmetic as pano_reproject_to16s (the reference GPU
  * samples bicubic, CudaReproject.cu:714-741): dst is device
@@ -282,6 +289,10 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
 int pano_render_render_to(pano_render* r, int num, const uint8_t* const* srcs, const size_t* src_steps,
                           const uint8_t* luts, const pano_output* out, hipStream_t stream);
 
+/* Host wall time of the last pano_render_prepare_maps, in ms: map support
+ * scan, seams (on the device), host tables (weights, work lists), device
+ * build (uploads, fused maps). */
+int pano_render_prepare_ms(const pano_render* r, double* ms4);
 int pano_render_num_images(const pano_render* r);
 int pano_render_num_levels(const pano_render* r);
 int64_t pano_render_mem_bytes(const pano_render* r);

@@ -79,6 +79,8 @@ def lib():
             "pano_blender_mem_bytes": ([vp], i64),
             "pano_reproject": ([vp, sz, i, i, vp, vp, sz, vp, sz, i, i, vp], i),
             "pano_linear_weights": ([i, i, i, pp, psz, i, i, pp, psz], i),
+            "pano_nonintersecting_masks": ([i, i, i, pp, psz, pp, psz], i),
+            "pano_render_prepare_ms": ([vp, vp], i),
             "pano_linear_blender_create": ([pp], i),
             "pano_linear_blender_destroy": ([vp], None),
             "pano_linear_blender_prepare": ([vp, i, i, i, pp, psz, i], i),
@@ -178,6 +180,7 @@ def exported_symbols():
         "pano_linear_blender_create", "pano_linear_blender_destroy", "pano_linear_blender_prepare",
         "pano_linear_blender_blend", "pano_linear_render_create", "pano_linear_render_destroy",
         "pano_linear_render_prepare_maps", "pano_linear_render_render", "pano_linear_render_weights",
+        "pano_nonintersecting_masks", "pano_render_prepare_ms",
     ]
 
 
@@ -396,6 +399,18 @@ class CudaTilingMultibandBlendFast:
     @staticmethod
     def estimate_memory_size(width: int, height: int, num_images: int) -> int:
         return lib().pano_estimate_mem_bytes(width, height, num_images)
+
+
+def nonintersecting_masks(masks) -> np.ndarray:
+    """getNonIntersectingMasks (Blend/Mask.cpp:376-397) on the device: host
+    (n, h, w) uint8 masks -> host (n, h, w) unique masks."""
+    masks = np.ascontiguousarray(masks, np.uint8)
+    n, h, w = masks.shape
+    out = np.zeros_like(masks)
+    _check(lib().pano_nonintersecting_masks(n, h, w, _ptrs([m.ctypes.data for m in masks]), _sizes([w] * n),
+                                            _ptrs([o.ctypes.data for o in out]), _sizes([w] * n)),
+           "getNonIntersectingMasks")
+    return out
 
 
 def linear_weights(masks, radius: int, as_int: bool = False):
@@ -759,6 +774,12 @@ class PanoramaRender:
             return False
         _check(st, "render")
         return True
+
+    def prepare_ms(self):
+        """Host wall time of the last prepare_maps by phase (ms)."""
+        ms = np.zeros(4, np.float64)
+        _check(lib().pano_render_prepare_ms(self._h, C.c_void_p(ms.ctypes.data)), "prepare_ms")
+        return dict(zip(("support", "seams", "tables", "build"), ms.round(1).tolist()))
 
     def render_to(self, srcs, output: "Output", luts=None, stream=None) -> bool:
         """render() into an Output (the canvas, or NV12 / YUV420P planes)."""

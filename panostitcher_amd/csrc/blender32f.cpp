@@ -109,7 +109,8 @@ int pano_blender32f_prepare(pano_blender32f* b, int num, int rows, int cols,
         b->lh[j] = (b->lh[j - 1] + 1) / 2;
     }
     std::vector<uint8_t> unique(P * num);
-    nonintersecting_masks(num, rows, cols, masks, mask_steps, unique.data());
+    if (device_nonintersecting_masks(num, rows, cols, masks, mask_steps, unique.data()) != hipSuccess)
+        return PANO_ERR_HIP;
     std::vector<uint8_t> any(P, 0);
     for (int i = 0; i < num; i++)
         for (int y = 0; y < rows; y++)

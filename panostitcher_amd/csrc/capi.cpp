@@ -4,6 +4,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <memory>
 #include <vector>
 
@@ -191,6 +193,19 @@ bool plane_ok(const uint8_t* p, size_t step, size_t row, int rows, size_t align)
 {
     return p && step >= row && step % align == 0 && (uintptr_t)p % align == 0 && step < (1u << 24) &&
            step * (uint64_t)rows <= 0xffffffffu;
+}
+
+// getNonIntersectingMasks on the device for a prepare (argument errors first,
+// as prepare_rig reports them).
+int device_seams(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps,
+                 std::vector<uint8_t>& out)
+{
+    if (n <= 0 || n > 255 || rows <= 0 || cols <= 0 || !masks || !steps) return PANO_ERR_ARG;
+    for (int i = 0; i < n; i++)
+        if (!masks[i] || steps[i] < (size_t)cols) return PANO_ERR_ARG;
+    out.resize((size_t)rows * cols * n);
+    HIP_TRY(device_nonintersecting_masks(n, rows, cols, masks, steps, out.data()));
+    return PANO_OK;
 }
 
 // One launch of the fused 8-bit sequence: a stage over a device segment table.
@@ -762,6 +777,7 @@ struct pano_render {
     int lut_state = 0;                // the device's "apply LUTs" byte
     std::vector<uint8_t> lut_block;   // n x 768 LUTs + the flag byte
     uint8_t* bgrx_scratch = nullptr;  // 16-bit path with encoder-plane output: the BGRx canvas
+    double prep_ms[4] = {};           // last prepare: map support, seams, host tables, device build
     ~pano_render()
     {
         if (bgrx_scratch) (void)hipFree(bgrx_scratch);
@@ -910,8 +926,11 @@ int pano_blender_prepare(pano_blender* b, int num, int rows, int cols, const uin
     b->rig.reset(new (std::nothrow) DeviceRig());
     if (!b->rig) return PANO_ERR_NOMEM;
     b->rig->kind = kPixS16;
-    int err = prepare_rig(num, rows, cols, masks, mask_steps, nullptr, max_levels, min_length,
-                          true, kPixS16, false, b->rig->prep);
+    std::vector<uint8_t> unique;
+    int err = device_seams(num, rows, cols, masks, mask_steps, unique);
+    if (err) return err;
+    err = prepare_rig(num, rows, cols, masks, mask_steps, nullptr, max_levels, min_length, true, kPixS16,
+                      false, b->rig->prep, unique.data());
     if (err) return err;
     err = b->rig->build(true, true, nullptr, nullptr, masks, mask_steps, nullptr, nullptr);
     if (err) return err;
@@ -1099,6 +1118,21 @@ int pano_reproject_to16s(const uint8_t* src, size_t src_step, int src_w, int src
     return PANO_OK;
 }
 
+int pano_nonintersecting_masks(int num, int rows, int cols, const uint8_t* const* masks, const size_t* mask_steps,
+                               uint8_t* const* out, const size_t* out_steps)
+{
+    if (!out || !out_steps) return PANO_ERR_ARG;
+    std::vector<uint8_t> unique;
+    int e = device_seams(num, rows, cols, masks, mask_steps, unique);
+    if (e) return e;
+    for (int i = 0; i < num; i++) {
+        if (!out[i] || out_steps[i] < (size_t)cols) return PANO_ERR_ARG;
+        for (int y = 0; y < rows; y++)
+            memcpy(out[i] + (size_t)y * out_steps[i], unique.data() + ((size_t)i * rows + y) * cols, cols);
+    }
+    return PANO_OK;
+}
+
 int pano_reproject(const uint8_t* src, size_t src_step, int src_w, int src_h, const float* xmap,
                    const float* ymap, size_t map_step, uint8_t* dst, size_t dst_step, int dst_w, int dst_h,
                    hipStream_t stream)
@@ -1226,24 +1260,30 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
     if (max_levels <= 0) return PANO_ERR_ARG;
     const size_t P = (size_t)dst_w * dst_h;
     if (P * 4 > 0xffffffffu) return PANO_ERR_UNSUPPORTED;   // 32-bit canvas offsets
+    using clock = std::chrono::steady_clock;
+    const auto t0 = clock::now();
     std::vector<size_t> steps(num, (size_t)dst_w);
     // pixels whose level-0 value can be non-zero: mask or a valid map point
     std::vector<std::vector<uint8_t>> support(num, std::vector<uint8_t>(P));
     std::vector<const uint8_t*> sp(num);
-    bool consistent = true;
     for (int i = 0; i < num; i++) {
         if (!xmaps[i] || !ymaps[i] || !masks[i] || src_w[i] <= 0 || src_h[i] <= 0) return PANO_ERR_ARG;
         // 24-bit row offsets in the remap, 32-bit frame offsets
         if (lut_step(src_w[i]) >= (1u << 24) || lut_step(src_w[i]) * (uint64_t)src_h[i] > 0xffffffffu)
             return PANO_ERR_UNSUPPORTED;
+        sp[i] = support[i].data();
+    }
+    std::atomic<bool> consistent{true};
+    parallel_for(num, [&](int i) {
+        bool ok = true;
         for (size_t p = 0; p < P; p++) {
             double x = xmaps[i][p], y = ymaps[i][p];
             bool v = x >= 0 && y >= 0 && x < src_w[i] && y < src_h[i];
             support[i][p] = v;
-            if (v && !masks[i][p]) consistent = false;
+            if (v && !masks[i][p]) ok = false;
         }
-        sp[i] = support[i].data();
-    }
+        if (!ok) consistent = false;
+    });
     r->rig.reset(new (std::nothrow) DeviceRig());
     if (!r->rig) return PANO_ERR_NOMEM;
     // 8-bit planes hold every Gaussian level exactly when sampled pixels lie
@@ -1251,11 +1291,25 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
     // reference's pyramid can leave [0, 255] and 16-bit planes are used.
     const bool fused = consistent && true_num_levels(dst_w, dst_h, max_levels, 2) >= 1;
     r->rig->kind = fused ? kPixU8 : kPixS16;
-    int err = prepare_rig(num, dst_h, dst_w, masks, steps.data(), sp.data(), max_levels, 2, false,
-                          r->rig->kind, fused, r->rig->prep);
+    const auto t1 = clock::now();
+    std::vector<uint8_t> unique;
+    int err = device_seams(num, dst_h, dst_w, masks, steps.data(), unique);
     if (err) return err;
+    const auto t2 = clock::now();
+    err = prepare_rig(num, dst_h, dst_w, masks, steps.data(), sp.data(), max_levels, 2, false, r->rig->kind,
+                      fused, r->rig->prep, unique.data());
+    if (err) return err;
+    const auto t3 = clock::now();
     err = r->rig->build(true, false, xmaps, ymaps, masks, steps.data(), src_w, src_h);
     if (err) return err;
+    const auto t4 = clock::now();
+    auto ms = [](clock::time_point a, clock::time_point b) {
+        return std::chrono::duration<double, std::milli>(b - a).count();
+    };
+    r->prep_ms[0] = ms(t0, t1);
+    r->prep_ms[1] = ms(t1, t2);
+    r->prep_ms[2] = ms(t2, t3);
+    r->prep_ms[3] = ms(t3, t4);
     err = r->ring.init((size_t)num * 768 + 1);
     if (err) return err;
     r->src_w.assign(src_w, src_w + num);
@@ -1492,6 +1546,13 @@ int pano_render_stage_ms(pano_render* r, double* ms4, int* frames)
 }
 
 int pano_render_fused(const pano_render* r) { return r && r->ok && r->rig->prep.fused0 ? 1 : 0; }
+
+int pano_render_prepare_ms(const pano_render* r, double* ms4)
+{
+    if (!r || !ms4) return PANO_ERR_ARG;
+    for (int k = 0; k < 4; k++) ms4[k] = r->prep_ms[k];
+    return PANO_OK;
+}
 
 int pano_render_num_images(const pano_render* r) { return r && r->ok ? r->rig->prep.n : 0; }
 int pano_render_num_levels(const pano_render* r) { return r && r->ok ? r->rig->prep.levels : -1; }

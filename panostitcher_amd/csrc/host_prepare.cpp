@@ -32,23 +32,6 @@ int reflect_idx(int p, int n)
     return p;
 }
 
-template <class F>
-void parallel_for(int n, F f)
-{
-    int hw = (int)std::thread::hardware_concurrency();
-    int nt = std::max(1, std::min(n, std::min(hw > 0 ? hw : 1, 16)));
-    if (nt <= 1) {
-        for (int i = 0; i < n; i++) f(i);
-        return;
-    }
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; t++)
-        th.emplace_back([&, t]() {
-            for (int i = t; i < n; i += nt) f(i);
-        });
-    for (auto& x : th) x.join();
-}
-
 // Smallest cyclic interval [x0, x0 + len) of [0, n) containing every set flag.
 void cyclic_interval(const std::vector<uint8_t>& f, int n, int& x0, int& len)
 {
@@ -173,7 +156,7 @@ int true_num_levels(int width, int height, int max_levels, int min_length)
 
 int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps,
                 const uint8_t* const* support0, int max_levels, int min_length, bool full_boxes,
-                int kind, bool fused0, Prepared& out)
+                int kind, bool fused0, Prepared& out, const uint8_t* unique)
 {
     if (n <= 0 || n > 255 || rows <= 0 || cols <= 0 || !masks || !steps) return PANO_ERR_ARG;
     for (int i = 0; i < n; i++)
@@ -188,7 +171,10 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
 
     // seams and owners
     out.unique.resize(P * n);
-    nonintersecting_masks(n, rows, cols, masks, steps, out.unique.data());
+    if (unique)
+        std::copy(unique, unique + P * n, out.unique.begin());
+    else
+        nonintersecting_masks(n, rows, cols, masks, steps, out.unique.data());
     out.label.assign(P, kNoOwner);
     for (int i = 0; i < n; i++) {
         const uint8_t* u = out.unique.data() + P * i;

@@ -8,8 +8,8 @@
 //                            1269-1287, 1543-1558): weights at prepare, then per
 //                            frame one launch that samples every camera listed for
 //                            a canvas tile and sums the weighted samples.
-// The seams come from the host (getNonIntersectingMasks); the Gaussian blur,
-// the weights and the per-frame work run on the device.
+// The seams (getNonIntersectingMasks), the Gaussian blur, the weights and the
+// per-frame work all run on the device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -98,7 +98,7 @@ int device_weights(int n, int rows, int cols, const uint8_t* const* masks, const
 {
     const size_t P = (size_t)rows * pitch;
     std::vector<uint8_t> unique((size_t)rows * cols * n), host(P * n, 0), uhost(P * n, 0);
-    nonintersecting_masks(n, rows, cols, masks, steps, unique.data());
+    if (device_nonintersecting_masks(n, rows, cols, masks, steps, unique.data()) != hipSuccess) return PANO_ERR_HIP;
     for (int i = 0; i < n; i++)
         for (int y = 0; y < rows; y++) {
             memcpy(&host[P * i + (size_t)y * pitch], masks[i] + (size_t)y * steps[i], cols);

@@ -4,11 +4,31 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "pano_dev.hpp"
 
 namespace pano {
+
+// f(0) .. f(n - 1) on up to 16 host threads.
+template <class F>
+void parallel_for(int n, F f)
+{
+    int hw = (int)std::thread::hardware_concurrency();
+    int nt = std::max(1, std::min(n, std::min(hw > 0 ? hw : 1, 16)));
+    if (nt <= 1) {
+        for (int i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++)
+        th.emplace_back([&, t]() {
+            for (int i = t; i < n; i += nt) f(i);
+        });
+    for (auto& x : th) x.join();
+}
 
 // Geometry of one camera plane at one level (a box of the level canvas).
 struct Geom {
@@ -55,8 +75,9 @@ int true_num_levels(int width, int height, int max_levels, int min_length);
 // kind selects the work-list geometry (kPixU8: fused 8-bit kernels, quad
 // tiles; kPixS16: generic kernels); fused0: the 8-bit render path, whose
 // level 0 -> 1 jobs address cameras instead of planes.
+// unique (optional): the seams already computed (n planes rows x cols).
 int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps,
                 const uint8_t* const* support0, int max_levels, int min_length, bool full_boxes,
-                int kind, bool fused0, Prepared& out);
+                int kind, bool fused0, Prepared& out, const uint8_t* unique = nullptr);
 
 }  // namespace pano

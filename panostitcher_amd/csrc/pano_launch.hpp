@@ -105,6 +105,11 @@ hipError_t launch_linear_render(const LinTile* tiles, int ntiles, const int* cam
 hipError_t launch_linear_blend32s(const SrcTable& images, int n, const int32_t* wts, int pitch, size_t cam_stride,
                                   uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
 
+// getNonIntersectingMasks on the device (seams.hip): host masks in, host
+// unique masks out (n planes rows x cols); synchronous.
+hipError_t device_nonintersecting_masks(int n, int rows, int cols, const uint8_t* const* masks,
+                                        const size_t* steps, uint8_t* out);
+
 hipError_t launch_copy_plane(const void* src, size_t src_step, void* dst, size_t dst_step,
                              int row_bytes, int rows, hipStream_t s);
 

@@ -60,6 +60,20 @@ def test_reproject_to16s_matches_oracle():
     assert not got8[..., 3].any()
 
 
+@pytest.mark.parametrize("seed,n,h,w", [(0, 3, 40, 96), (1, 4, 17, 2000), (2, 1, 9, 13), (3, 5, 64, 300),
+                                        (4, 3, 1, 77)])
+def test_nonintersecting_masks_device_matches_oracle(seed, n, h, w):
+    rng = np.random.default_rng(seed)
+    masks = random_masks(rng, n, h, w) if h > 3 else (rng.random((n, h, w)) < 0.7).astype(np.uint8) * 255
+    if n > 2:
+        masks[1][:, :] = 200        # no zero pixel: the distance is the 8192 frame cap
+        masks[2][masks[2] == 0] = 0
+        masks[2][(rng.random((h, w)) < 0.1)] = 0
+    note(f"seams on the device seed={seed} n={n} {w}x{h}")
+    got = P.nonintersecting_masks(masks)
+    np.testing.assert_array_equal(got, O.nonintersecting_masks(masks))
+
+
 def test_transform_matches_oracle():
     rng = np.random.default_rng(2)
     src = rng.integers(0, 256, size=(19, 33, 4), dtype=np.uint8)
