@@ -255,6 +255,16 @@ int pano_exposure_luts(int num, const double* exposures, const double* red_ratio
 int pano_generate_map(const pano_photo_param* param, int src_w, int src_h, int dst_w, int dst_h,
                       float* xmap, float* ymap, uint8_t* mask);
 
+/* cudaGenerateReprojectMapAndMask(param, srcSize, dstSize, xmap, ymap, mask)
+ * (CudaInterface.h:129-130, CudaReproject.cu:266-385): the same FP64 chain on the
+ * device, one canvas pixel per thread.  xmap / ymap: device float planes of
+ * dst_w x dst_h (row step map_step bytes), mask: device bytes (mask_step).
+ * The device's FP64 sin / cos / atan2 / tan may differ from the host library's
+ * in the last bits; pano_render_prepare generates its maps this way. */
+int pano_generate_map_device(const pano_photo_param* param, int src_w, int src_h, int dst_w, int dst_h,
+                             float* xmap, float* ymap, size_t map_step, uint8_t* mask, size_t mask_step,
+                             hipStream_t stream);
+
 /* ---- render: CudaPanoramaRender2 (Task/RicohUtil.h:134-161) -------------
  * prepare + per-frame render of one rig: maps, masks, seams and weight
  * pyramids stay resident on the device; render() runs the fused path

@@ -81,6 +81,7 @@ def lib():
             "pano_linear_weights": ([i, i, i, pp, psz, i, i, pp, psz], i),
             "pano_nonintersecting_masks": ([i, i, i, pp, psz, pp, psz], i),
             "pano_render_prepare_ms": ([vp, vp], i),
+            "pano_generate_map_device": ([vp, i, i, i, i, vp, vp, sz, vp, sz, vp], i),
             "pano_linear_blender_create": ([pp], i),
             "pano_linear_blender_destroy": ([vp], None),
             "pano_linear_blender_prepare": ([vp, i, i, i, pp, psz, i], i),
@@ -180,7 +181,7 @@ def exported_symbols():
         "pano_linear_blender_create", "pano_linear_blender_destroy", "pano_linear_blender_prepare",
         "pano_linear_blender_blend", "pano_linear_render_create", "pano_linear_render_destroy",
         "pano_linear_render_prepare_maps", "pano_linear_render_render", "pano_linear_render_weights",
-        "pano_nonintersecting_masks", "pano_render_prepare_ms",
+        "pano_nonintersecting_masks", "pano_render_prepare_ms", "pano_generate_map_device",
     ]
 
 
@@ -278,6 +279,20 @@ def exposure_luts(exposures, red_ratios, blue_ratios) -> np.ndarray:
     _check(lib().pano_exposure_luts(len(e), e.ctypes.data, r.ctypes.data, b.ctypes.data,
                                     out.ctypes.data), "getExposureColorOptimizeLUTs")
     return out
+
+
+def generate_map_device(param: PhotoParam, src_w: int, src_h: int, dst_w: int, dst_h: int, stream=None):
+    """cudaGenerateReprojectMapAndMask: (xmap, ymap, mask) cuda tensors of dst_h x dst_w."""
+    import torch
+
+    xm = torch.empty((dst_h, dst_w), dtype=torch.float32, device="cuda")
+    ym = torch.empty_like(xm)
+    mk = torch.empty((dst_h, dst_w), dtype=torch.uint8, device="cuda")
+    _check(lib().pano_generate_map_device(C.byref(param), src_w, src_h, dst_w, dst_h,
+                                          C.c_void_p(xm.data_ptr()), C.c_void_p(ym.data_ptr()), dst_w * 4,
+                                          C.c_void_p(mk.data_ptr()), dst_w, _stream(stream)),
+           "cudaGenerateReprojectMapAndMask")
+    return xm, ym, mk
 
 
 def generate_map(param: PhotoParam, src_w: int, src_h: int, dst_w: int, dst_h: int):

@@ -16,6 +16,8 @@
 namespace pano {
 int generate_map(const pano_photo_param& p, int src_w, int src_h, int dst_w, int dst_h,
                  float* xmap, float* ymap, uint8_t* mask);
+int generate_map_device(const pano_photo_param& p, int src_w, int src_h, int dst_w, int dst_h, float* xmap,
+                        float* ymap, size_t map_step, uint8_t* mask, size_t mask_step, hipStream_t s);
 }
 
 using namespace pano;
@@ -1320,6 +1322,14 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
     return PANO_OK;
 }
 
+int pano_generate_map_device(const pano_photo_param* param, int src_w, int src_h, int dst_w, int dst_h,
+                             float* xmap, float* ymap, size_t map_step, uint8_t* mask, size_t mask_step,
+                             hipStream_t stream)
+{
+    if (!param) return PANO_ERR_ARG;
+    return generate_map_device(*param, src_w, src_h, dst_w, dst_h, xmap, ymap, map_step, mask, mask_step, stream);
+}
+
 int pano_render_prepare(pano_render* r, int num, const pano_photo_param* params, const int* src_w,
                         const int* src_h, int dst_w, int dst_h, int max_levels)
 {
@@ -1330,14 +1340,30 @@ int pano_render_prepare(pano_render* r, int num, const pano_photo_param* params,
     std::vector<std::vector<uint8_t>> mk(num, std::vector<uint8_t>(P));
     std::vector<const float*> xp(num), yp(num);
     std::vector<const uint8_t*> kp(num);
-    for (int i = 0; i < num; i++) {
-        int e = generate_map(params[i], src_w[i], src_h[i], dst_w, dst_h, xm[i].data(),
-                             ym[i].data(), mk[i].data());
-        if (e) return e;
+    // maps and masks on the device (cudaGenerateReprojectMapsAndMasks), read back
+    // for the host tables of the prepare
+    float* dmap = nullptr;
+    uint8_t* dmask = nullptr;
+    if (hipMalloc((void**)&dmap, P * 8) != hipSuccess) return PANO_ERR_NOMEM;
+    if (hipMalloc((void**)&dmask, P) != hipSuccess) {
+        (void)hipFree(dmap);
+        return PANO_ERR_NOMEM;
+    }
+    int e = PANO_OK;
+    for (int i = 0; i < num && !e; i++) {
+        e = generate_map_device(params[i], src_w[i], src_h[i], dst_w, dst_h, dmap, dmap + P, (size_t)dst_w * 4,
+                                dmask, (size_t)dst_w, nullptr);
+        if (!e && (hipMemcpy(xm[i].data(), dmap, P * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+                   hipMemcpy(ym[i].data(), dmap + P, P * 4, hipMemcpyDeviceToHost) != hipSuccess ||
+                   hipMemcpy(mk[i].data(), dmask, P, hipMemcpyDeviceToHost) != hipSuccess))
+            e = PANO_ERR_HIP;
         xp[i] = xm[i].data();
         yp[i] = ym[i].data();
         kp[i] = mk[i].data();
     }
+    (void)hipFree(dmap);
+    (void)hipFree(dmask);
+    if (e) return e;
     return pano_render_prepare_maps(r, num, src_w, src_h, dst_w, dst_h, xp.data(), yp.data(),
                                     kp.data(), max_levels);
 }

@@ -1,8 +1,11 @@
-// maps.cpp -- equirect -> camera coordinate maps on the host
-// (getReprojectMap32FAndMask, Warp/ZReproject.cpp:132-240, over the PanoTools
-// transform chain of Warp/Remap.cpp:147-514).  FP64 throughout; rows are
-// computed in parallel.  The reference names the equirect canvas "src" and the
-// camera "dst" inside Remap (ZReproject.h:66-71); here canvas and camera.
+// maps.cpp -- equirect -> camera coordinate maps (getReprojectMap32FAndMask,
+// Warp/ZReproject.cpp:132-240, over the PanoTools transform chain of
+// Warp/Remap.cpp:147-514), on the host (rows in parallel) or on the device
+// (cudaGenerateReprojectMapAndMask, CudaAccel/CudaReproject.cu:266-385): the
+// same FP64 chain, one point per canvas pixel.  The reference names the
+// equirect canvas "src" and the camera "dst" inside Remap (ZReproject.h:66-71);
+// here canvas and camera.
+#include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
 
@@ -16,7 +19,7 @@ namespace pano {
 
 namespace {
 
-const double kPi = 3.14159265358979323846264338327950288;
+constexpr double kPi = 3.14159265358979323846264338327950288;
 inline double deg2rad(double d) { return d * 2.0 * kPi / 360.0; }
 
 struct CameraChain {
@@ -137,7 +140,7 @@ void chain_init(CameraChain& c, const pano_photo_param& p0, int canvasW, int can
 }
 
 // Remap::remapImage (Remap.cpp:356-514): canvas pixel -> camera coordinates.
-void chain_apply(const CameraChain& c, double cx, double cy, double& sx, double& sy)
+__host__ __device__ void chain_apply(const CameraChain& c, double cx, double cy, double& sx, double& sy)
 {
     cx -= c.canvasTX - 0.5;
     cy -= c.canvasTY - 0.5;
@@ -187,6 +190,71 @@ void chain_apply(const CameraChain& c, double cx, double cy, double& sx, double&
     sy = y + (c.camTY - 0.5);
 }
 
+// The chain and the validity test of one camera (getReprojectMapAndMask,
+// ZReproject.cpp:152 and :173-199): inside the frame, or for circular / drum
+// fisheyes inside the crop rectangle and the circle.
+struct MapSetup {
+    CameraChain c;
+    bool circ = false;
+    double cx = 0, cy = 0, r2 = 0;
+    int cropX = 0, cropY = 0, cropW = 0, cropH = 0, src_w = 0, src_h = 0;
+};
+
+MapSetup map_setup(const pano_photo_param& p, int src_w, int src_h, int dst_w, int dst_h)
+{
+    MapSetup m;
+    chain_init(m.c, p, dst_w, dst_h, src_w, src_h);
+    m.circ = p.imageType == 3 || p.imageType == 2;
+    m.cropX = p.cropX;
+    m.cropY = p.cropY;
+    m.cropW = p.cropWidth;
+    m.cropH = p.cropHeight;
+    m.src_w = src_w;
+    m.src_h = src_h;
+    if (m.circ) {
+        if (p.circleR > 0) {
+            m.cx = p.circleX;
+            m.cy = p.circleY;
+            m.r2 = (double)p.circleR * p.circleR;
+        } else {
+            m.cx = p.cropX + p.cropWidth / 2;
+            m.cy = p.cropY + p.cropHeight / 2;
+            m.r2 = p.cropWidth > p.cropHeight ? p.cropWidth * p.cropWidth * 0.25 : p.cropHeight * p.cropHeight * 0.25;
+        }
+    }
+    return m;
+}
+
+// Canvas pixel (w, h): the map point (float, -1 where invalid) and the mask byte.
+__host__ __device__ void map_point(const MapSetup& m, int w, int h, float& x, float& y, uint8_t& mask)
+{
+    double sx, sy;
+    chain_apply(m.c, w, h, sx, sy);
+    bool ok;
+    if (!m.circ)
+        ok = sx >= 0 && sy >= 0 && sx < m.src_w && sy < m.src_h;
+    else
+        ok = sx >= m.cropX && sy >= m.cropY && sx < m.cropX + m.cropW && sy < m.cropY + m.cropH &&
+             (sx - m.cx) * (sx - m.cx) + (sy - m.cy) * (sy - m.cy) < m.r2;
+    mask = ok ? 255 : 0;
+    x = ok ? (float)sx : -1.f;
+    y = ok ? (float)sy : -1.f;
+}
+
+__global__ __launch_bounds__(256) void k_generate_map(MapSetup m, int dst_w, int dst_h, float* __restrict__ xmap,
+                                                      float* __restrict__ ymap, size_t map_step,
+                                                      uint8_t* __restrict__ mask, size_t mask_step)
+{
+    const int w = blockIdx.x * 64 + (threadIdx.x & 63), h = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (w >= dst_w || h >= dst_h) return;
+    float x, y;
+    uint8_t k;
+    map_point(m, w, h, x, y, k);
+    ((float*)((char*)xmap + (size_t)h * map_step))[w] = x;
+    ((float*)((char*)ymap + (size_t)h * map_step))[w] = y;
+    mask[(size_t)h * mask_step + w] = k;
+}
+
 }  // namespace
 
 int generate_map(const pano_photo_param& p, int src_w, int src_h, int dst_w, int dst_h,
@@ -194,22 +262,7 @@ int generate_map(const pano_photo_param& p, int src_w, int src_h, int dst_w, int
 {
     if (src_w <= 0 || src_h <= 0 || dst_w <= 0 || dst_h <= 0 || !xmap || !ymap || !mask)
         return PANO_ERR_ARG;
-    CameraChain c;
-    chain_init(c, p, dst_w, dst_h, src_w, src_h);
-    const bool circ = p.imageType == 3 || p.imageType == 2;
-    double cx = 0, cy = 0, r2 = 0;
-    if (circ) {
-        if (p.circleR > 0) {
-            cx = p.circleX;
-            cy = p.circleY;
-            r2 = (double)p.circleR * p.circleR;
-        } else {
-            cx = p.cropX + p.cropWidth / 2;
-            cy = p.cropY + p.cropHeight / 2;
-            r2 = p.cropWidth > p.cropHeight ? p.cropWidth * p.cropWidth * 0.25
-                                            : p.cropHeight * p.cropHeight * 0.25;
-        }
-    }
+    const MapSetup m = map_setup(p, src_w, src_h, dst_w, dst_h);
     int hw = (int)std::thread::hardware_concurrency();
     int nt = std::max(1, std::min(hw > 0 ? hw : 1, 32));
     std::vector<std::thread> th;
@@ -217,23 +270,26 @@ int generate_map(const pano_photo_param& p, int src_w, int src_h, int dst_w, int
         th.emplace_back([&, t]() {
             for (int h = t; h < dst_h; h += nt)
                 for (int w = 0; w < dst_w; w++) {
-                    double sx, sy;
-                    chain_apply(c, w, h, sx, sy);
-                    bool ok;
-                    if (!circ)
-                        ok = sx >= 0 && sy >= 0 && sx < src_w && sy < src_h;
-                    else
-                        ok = sx >= p.cropX && sy >= p.cropY && sx < p.cropX + p.cropWidth &&
-                             sy < p.cropY + p.cropHeight &&
-                             (sx - cx) * (sx - cx) + (sy - cy) * (sy - cy) < r2;
                     size_t o = (size_t)h * dst_w + w;
-                    mask[o] = ok ? 255 : 0;
-                    xmap[o] = ok ? (float)sx : -1.f;
-                    ymap[o] = ok ? (float)sy : -1.f;
+                    map_point(m, w, h, xmap[o], ymap[o], mask[o]);
                 }
         });
     for (auto& x : th) x.join();
     return PANO_OK;
+}
+
+// The same on the device: xmap / ymap / mask are device planes with the given
+// row steps (bytes).
+int generate_map_device(const pano_photo_param& p, int src_w, int src_h, int dst_w, int dst_h, float* xmap,
+                        float* ymap, size_t map_step, uint8_t* mask, size_t mask_step, hipStream_t s)
+{
+    if (src_w <= 0 || src_h <= 0 || dst_w <= 0 || dst_h <= 0 || !xmap || !ymap || !mask ||
+        map_step < (size_t)dst_w * 4 || map_step % 4 || mask_step < (size_t)dst_w)
+        return PANO_ERR_ARG;
+    const MapSetup m = map_setup(p, src_w, src_h, dst_w, dst_h);
+    hipLaunchKernelGGL(k_generate_map, dim3((dst_w + 63) / 64, (dst_h + 3) / 4), dim3(256), 0, s, m, dst_w, dst_h,
+                       xmap, ymap, map_step, mask, mask_step);
+    return hipGetLastError() == hipSuccess ? PANO_OK : PANO_ERR_HIP;
 }
 
 }  // namespace pano
