@@ -222,6 +222,23 @@ int pano_reproject(const uint8_t* src, size_t src_step, int src_w, int src_h, co
                    const float* ymap, size_t map_step, uint8_t* dst, size_t dst_step, int dst_w, int dst_h,
                    hipStream_t stream);
 
+/* cudaReprojectWeightedAccumulateTo32F(src, dst, xmap, ymap, weight, stream)
+ * (CudaInterface.h:153-155) with the CPU path's bilinear samples and float
+ * arithmetic (Warp/ZReproject.cpp:606-631): where the map is valid, accum
+ * (device float BGRx, dst_w x dst_h) += sample x weight per channel and
+ * channel 3 is set to 0.  weight: device float plane. */
+int pano_reproject_weighted_accumulate(const uint8_t* src, size_t src_step, int src_w, int src_h, const float* xmap,
+                                       const float* ymap, size_t map_step, const float* weight, size_t weight_step,
+                                       float* accum, size_t accum_step, int dst_w, int dst_h, hipStream_t stream);
+
+/* cudaReproject / cudaReprojectTo16S(src, dst, [mask,] dstSize, param, stream)
+ * (CudaInterface.h:135-145): the map generated on the device for this call,
+ * then the bilinear reprojection (type PANO_8UC4 or PANO_16SC4); mask (device,
+ * optional) receives the validity mask. */
+int pano_reproject_params(const uint8_t* src, size_t src_step, int src_w, int src_h, const pano_photo_param* param,
+                          int dst_w, int dst_h, int type, void* dst, size_t dst_step, uint8_t* mask,
+                          size_t mask_step, hipStream_t stream);
+
 /* cudaTransform(src, dst, luts, stream) (CudaInterface.h:120-121):
  * per-channel LUTs on BGRx (channel 3 copied).  luts: host, 3 x 256 bytes
  * (B, G, R).  src and dst may alias. */

@@ -82,6 +82,8 @@ def lib():
             "pano_nonintersecting_masks": ([i, i, i, pp, psz, pp, psz], i),
             "pano_render_prepare_ms": ([vp, vp], i),
             "pano_generate_map_device": ([vp, i, i, i, i, vp, vp, sz, vp, sz, vp], i),
+            "pano_reproject_weighted_accumulate": ([vp, sz, i, i, vp, vp, sz, vp, sz, vp, sz, i, i, vp], i),
+            "pano_reproject_params": ([vp, sz, i, i, vp, i, i, i, vp, sz, vp, sz, vp], i),
             "pano_linear_blender_create": ([pp], i),
             "pano_linear_blender_destroy": ([vp], None),
             "pano_linear_blender_prepare": ([vp, i, i, i, pp, psz, i], i),
@@ -182,6 +184,7 @@ def exported_symbols():
         "pano_linear_blender_blend", "pano_linear_render_create", "pano_linear_render_destroy",
         "pano_linear_render_prepare_maps", "pano_linear_render_render", "pano_linear_render_weights",
         "pano_nonintersecting_masks", "pano_render_prepare_ms", "pano_generate_map_device",
+        "pano_reproject_weighted_accumulate", "pano_reproject_params",
     ]
 
 
@@ -684,6 +687,37 @@ def cvt_yuv420p_to_bgr32(y, u, v, dst=None, stream=None):
                                        u.stride(0), C.c_void_p(v.data_ptr()), v.stride(0), w, h,
                                        C.c_void_p(dp), ds, _stream(stream)), "cvtYUV420PToBGR32")
     return dst
+
+
+def cuda_reproject_weighted_accumulate(src, accum, xmap, ymap, weight, stream=None):
+    """cudaReprojectWeightedAccumulateTo32F: accum (HxWx4 float32 cuda) += sample x weight."""
+    sh, sw = src.shape[:2]
+    dh, dw = xmap.shape
+    sp, ss = _dev_ptr_step(src)
+    ap, ast = _dev_ptr_step(accum)
+    _check(lib().pano_reproject_weighted_accumulate(C.c_void_p(sp), ss, sw, sh, C.c_void_p(xmap.data_ptr()),
+                                                    C.c_void_p(ymap.data_ptr()), xmap.stride(0) * 4,
+                                                    C.c_void_p(weight.data_ptr()), weight.stride(0) * 4,
+                                                    C.c_void_p(ap), ast, dw, dh, _stream(stream)),
+           "cudaReprojectWeightedAccumulateTo32F")
+    return accum
+
+
+def cuda_reproject_params(src, param, dst_size, to16s=False, with_mask=False, stream=None):
+    """cudaReproject / cudaReprojectTo16S(src, dst, [mask,] dstSize, param)."""
+    import torch
+
+    sh, sw = src.shape[:2]
+    dw, dh = dst_size
+    dst = torch.empty((dh, dw, 4), dtype=torch.int16 if to16s else torch.uint8, device=src.device)
+    mask = torch.empty((dh, dw), dtype=torch.uint8, device=src.device) if with_mask else None
+    sp, ss = _dev_ptr_step(src)
+    dp, ds = _dev_ptr_step(dst)
+    _check(lib().pano_reproject_params(C.c_void_p(sp), ss, sw, sh, C.byref(param), dw, dh,
+                                       PANO_16SC4 if to16s else PANO_8UC4, C.c_void_p(dp), ds,
+                                       C.c_void_p(mask.data_ptr()) if with_mask else None, dw if with_mask else 0,
+                                       _stream(stream)), "cudaReproject")
+    return (dst, mask) if with_mask else dst
 
 
 def cuda_transform(src, luts, dst=None, stream=None):

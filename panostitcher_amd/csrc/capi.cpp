@@ -1191,6 +1191,46 @@ int pano_nv12_to_bgr32(const uint8_t* y, size_t y_step, const uint8_t* uv, size_
     return PANO_OK;
 }
 
+int pano_reproject_weighted_accumulate(const uint8_t* src, size_t src_step, int src_w, int src_h, const float* xmap,
+                                       const float* ymap, size_t map_step, const float* weight, size_t weight_step,
+                                       float* accum, size_t accum_step, int dst_w, int dst_h, hipStream_t stream)
+{
+    if (!src || !xmap || !ymap || !weight || !accum || src_w <= 0 || src_h <= 0 || dst_w <= 0 || dst_h <= 0)
+        return PANO_ERR_ARG;
+    if (src_step < (size_t)src_w * 4 || src_step % 4 || src_step > 0xffffffffu || map_step < (size_t)dst_w * 4 ||
+        map_step % 4 || weight_step < (size_t)dst_w * 4 || weight_step % 4 || accum_step < (size_t)dst_w * 16 ||
+        accum_step % 16)
+        return PANO_ERR_ARG;
+    HIP_TRY(launch_reproject_accum(src, src_step, src_w, src_h, xmap, ymap, map_step, weight, weight_step, accum,
+                                   accum_step, dst_w, dst_h, stream));
+    return PANO_OK;
+}
+
+int pano_reproject_params(const uint8_t* src, size_t src_step, int src_w, int src_h, const pano_photo_param* param,
+                          int dst_w, int dst_h, int type, void* dst, size_t dst_step, uint8_t* mask,
+                          size_t mask_step, hipStream_t stream)
+{
+    if (!param || !dst || dst_w <= 0 || dst_h <= 0) return PANO_ERR_ARG;
+    if (type != PANO_8UC4 && type != PANO_16SC4) return PANO_ERR_ARG;
+    if (mask && mask_step < (size_t)dst_w) return PANO_ERR_ARG;
+    const size_t P = (size_t)dst_w * dst_h;
+    float* maps = nullptr;
+    uint8_t* m = mask;
+    HIP_TRY(hipMallocAsync((void**)&maps, P * 8 + (mask ? 0 : P), stream));
+    if (!m) m = (uint8_t*)(maps + 2 * P);
+    int e = generate_map_device(*param, src_w, src_h, dst_w, dst_h, maps, maps + P, (size_t)dst_w * 4, m,
+                                mask ? mask_step : (size_t)dst_w, stream);
+    if (!e)
+        e = type == PANO_8UC4 ? pano_reproject(src, src_step, src_w, src_h, maps, maps + P, (size_t)dst_w * 4,
+                                               (uint8_t*)dst, dst_step, dst_w, dst_h, stream)
+                              : pano_reproject_to16s(src, src_step, src_w, src_h, maps, maps + P,
+                                                     (size_t)dst_w * 4, (int16_t*)dst, dst_step, dst_w, dst_h,
+                                                     stream);
+    const hipError_t f = hipFreeAsync(maps, stream);
+    if (e) return e;
+    return f == hipSuccess ? PANO_OK : PANO_ERR_HIP;
+}
+
 int pano_transform_lut3(const uint8_t* src, size_t src_step, uint8_t* dst, size_t dst_step, int w,
                         int h, const uint8_t* luts, hipStream_t stream)
 {

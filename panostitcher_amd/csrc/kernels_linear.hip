@@ -144,9 +144,42 @@ __global__ __launch_bounds__(256) void k_linear_blend32s(SrcTable images, int n,
     stg((uint32_t*)(out + (size_t)Y * ostep) + X, o);
 }
 
+// reprojectWeightedAccumulateTo32F for one camera (free function): where the
+// map is valid, accum (float4) += bilinear sample x weight per channel (product
+// then sum, as Warp/ZReproject.cpp:606-631), channel 3 set to 0 (as
+// CudaReproject.cu:604-628).
+__global__ __launch_bounds__(256) void k_reproject_accum(const uint8_t* __restrict__ src, uint32_t sstep, int sw,
+                                                         int sh, const float* __restrict__ xmap,
+                                                         const float* __restrict__ ymap, size_t map_step,
+                                                         const float* __restrict__ wt, size_t wstep,
+                                                         float* __restrict__ acc, size_t astep, int w, int h)
+{
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63), Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (X >= w || Y >= h) return;
+    const double x = ((const float*)((const char*)xmap + (size_t)Y * map_step))[X];
+    const double y = ((const float*)((const char*)ymap + (size_t)Y * map_step))[X];
+    if (!(x >= 0 && y >= 0 && x < sw && y < sh)) return;
+    const uint32_t v = bilinear_bgr(src, sstep, sw, sh, x, y, nullptr);
+    const float wv = ((const float*)((const char*)wt + (size_t)Y * wstep))[X];
+    float* p = (float*)((char*)acc + (size_t)Y * astep) + 4 * X;
+    p[0] = __fadd_rn(p[0], __fmul_rn((float)(v & 255u), wv));
+    p[1] = __fadd_rn(p[1], __fmul_rn((float)((v >> 8) & 255u), wv));
+    p[2] = __fadd_rn(p[2], __fmul_rn((float)((v >> 16) & 255u), wv));
+    p[3] = 0.f;
+}
+
 namespace {
 dim3 grid_px(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }
 }  // namespace
+
+hipError_t launch_reproject_accum(const uint8_t* src, size_t sstep, int sw, int sh, const float* xmap,
+                                  const float* ymap, size_t map_step, const float* wt, size_t wstep, float* acc,
+                                  size_t astep, int w, int h, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reproject_accum, grid_px(w, h), dim3(256), 0, s, src, (uint32_t)sstep, sw, sh, xmap, ymap,
+                       map_step, wt, wstep, acc, astep, w, h);
+    return hipGetLastError();
+}
 
 hipError_t launch_blur_u8(const uint8_t* src, int pitch, int w, int h, const int* K, int n, int k0, int k1,
                           int32_t* rows, const uint8_t* mask, int mpitch, uint8_t* dst, hipStream_t s)

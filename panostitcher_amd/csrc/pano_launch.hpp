@@ -102,6 +102,9 @@ hipError_t launch_linear_weights(bool as_int, const uint8_t* dist, int pitch, si
                                  int h, void* out, hipStream_t s);
 hipError_t launch_linear_render(const LinTile* tiles, int ntiles, const int* camlist, const LinCam* cams,
                                 const SrcTable& src, uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
+hipError_t launch_reproject_accum(const uint8_t* src, size_t sstep, int sw, int sh, const float* xmap,
+                                  const float* ymap, size_t map_step, const float* wt, size_t wstep, float* acc,
+                                  size_t astep, int w, int h, hipStream_t s);
 hipError_t launch_linear_blend32s(const SrcTable& images, int n, const int32_t* wts, int pitch, size_t cam_stride,
                                   uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
 

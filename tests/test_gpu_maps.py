@@ -63,3 +63,35 @@ def test_render_prepared_from_params_matches_oracle():
     out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
     assert r.render([to_dev(synth.bgrx(s)) for s in srcs], out, luts)
     assert_parity(from_dev(out)[..., :3], ref, exact=True, label="render from params")
+
+
+def test_reproject_params_is_map_then_reproject():
+    wl = synth.workload("fisheye2", 0.25)
+    cam = wl.cameras[0]
+    src = to_dev(synth.bgrx(synth.frames(wl)[0]))
+    note("cudaReproject(src, dst, mask, dstSize, param)")
+    x, y, m = P.generate_map_device(cam.param(), cam.width, cam.height, wl.dst_w, wl.dst_h)
+    d8, mk = P.cuda_reproject_params(src, cam.param(), (wl.dst_w, wl.dst_h), with_mask=True)
+    np.testing.assert_array_equal(from_dev(mk), from_dev(m))
+    np.testing.assert_array_equal(from_dev(d8), from_dev(P.cuda_reproject(src, x, y)))
+    d16 = P.cuda_reproject_params(src, cam.param(), (wl.dst_w, wl.dst_h), to16s=True)
+    np.testing.assert_array_equal(from_dev(d16), from_dev(P.cuda_reproject_to16s(src, x, y)))
+
+
+def test_weighted_accumulate_matches_linear_render():
+    # accumulating every camera with the free function, then rounding, is the
+    # linear render (whose oracle parity is tested with the linear mode)
+    wl = synth.workload("rect6", 0.125)
+    xs, ys, ms = synth.maps_and_masks(wl)
+    srcs = synth.frames(wl)
+    w = O.linear_weights(np.stack(ms), 4)
+    ref = O.linear_render(srcs, xs, ys, w, None)
+    note("cudaReprojectWeightedAccumulateTo32F over rect6")
+    acc = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.float32, device="cuda")
+    for i in range(wl.n):
+        P.cuda_reproject_weighted_accumulate(to_dev(synth.bgrx(srcs[i])), acc, to_dev(xs[i]), to_dev(ys[i]),
+                                             to_dev(w[i]))
+    a = from_dev(acc)
+    got = np.clip(np.rint(a[..., :3]), 0, 255).astype(np.uint8)   # numpy rint: half to even, as convertTo
+    np.testing.assert_array_equal(got, ref)
+    assert not a[..., 3].any()
