@@ -106,7 +106,7 @@ __global__ __launch_bounds__(256) void k_linear_render(const LinTile* __restrict
         if (!(m.x >= 0.f && m.y >= 0.f && m.x < (float)C.sw && m.y < (float)C.sh)) continue;
         const uint8_t* sp = src.p[C.src];
         uint32_t o[4], p[4];
-        tap_offsets(src.step[C.src], C.sw, C.sh, m.x, m.y, o);
+        tap_offsets_fast(src.step[C.src], C.sw, C.sh, m.x, m.y, o);
 #pragma unroll
         for (int t = 0; t < 4; t++) p[t] = ldg((const uint32_t*)(sp + o[t]));
         bool tie = false;

@@ -237,7 +237,7 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
             const float x = __builtin_fabsf(m[cur][b].x), y = __builtin_fabsf(m[cur][b].y);
             px[b][0] = px[b][1] = px[b][2] = px[b][3] = 0;
             if (x < fsw && y < fsh) {
-                tap_offsets(sstep, cam.sw, cam.sh, x, y, o[b]);
+                tap_offsets_fast(sstep, cam.sw, cam.sh, x, y, o[b]);
 #pragma unroll
                 for (int q = 0; q < 4; q++) px[b][q] = ldg((const uint32_t*)(sp + o[b][q]));
             }

@@ -258,6 +258,24 @@ __device__ __forceinline__ void tap_offsets(uint32_t sstep, int sw, int sh, floa
     o[3] = o3;
 }
 
+// tap_offsets with the interior case first: away from the last row and column
+// the four samples are (x0, y0), (x0 + 1, y0) and the same one row down; the
+// edge rules of tap_offsets run only in lanes that need them.
+__device__ __forceinline__ void tap_offsets_fast(uint32_t sstep, int sw, int sh, float x, float y,
+                                                 uint32_t o[4])
+{
+    const int x0 = (int)x, y0 = (int)y;
+    if (x0 + 1 >= sw || y0 + 1 >= sh) {
+        tap_offsets(sstep, sw, sh, x, y, o);
+        return;
+    }
+    const uint32_t o0 = __umul24((uint32_t)y0, sstep) + 4u * (uint32_t)x0;
+    o[0] = o0;
+    o[1] = o0 + 4u;
+    o[2] = o0 + sstep;
+    o[3] = o0 + sstep + 4u;
+}
+
 // BT.601 studio swing in 20-bit fixed point: OpenCV's cvtColor table, which the
 // encoder hand-off uses (CudaAccel/ImageProc.cu:44-60).  Forward: Y of every
 // pixel of a 2x2 block, U and V from its left column (top and bottom averaged
