@@ -477,10 +477,11 @@ def run_linear(args):
         sampled += int(((w > 0) & (xs[i] >= 0) & (ys[i] >= 0) & (xs[i] < wl.cameras[i].width)
                         & (ys[i] < wl.cameras[i].height)).sum())
         m = ms[i] != 0
-        th, tw = (wl.dst_h + 3) // 4, (wl.dst_w + 63) // 64
-        pad = np.zeros((th * 4, tw * 64), bool)
+        TH = 16                                                # 64 x 16 canvas tiles
+        th, tw = (wl.dst_h + TH - 1) // TH, (wl.dst_w + 63) // 64
+        pad = np.zeros((th * TH, tw * 64), bool)
         pad[:wl.dst_h, :wl.dst_w] = m
-        listed += int(pad.reshape(th, 4, tw, 64).any(axis=(1, 3)).sum()) * 256
+        listed += int(pad.reshape(th, TH, tw, 64).any(axis=(1, 3)).sum()) * 64 * TH
     src_px = sum(c.width * c.height for c in wl.cameras)
     bytes_frame = 4 * listed + 8 * sampled + 3 * src_px + 4 * wl.out_pixels
     achieved = bytes_frame / (kms * 1e-3) / 1e9

@@ -88,38 +88,67 @@ __global__ __launch_bounds__(256) void k_linear_weights(const uint8_t* __restric
 }
 
 // One canvas tile of the linear render: cameras camlist[J.off .. J.off + J.cnt).
+// Each thread carries kLinRows pixels (rows Y, Y + 4, ...) through every step,
+// so that the loads of all of them are in flight together.
 __global__ __launch_bounds__(256) void k_linear_render(const LinTile* __restrict__ tiles,
                                                        const int* __restrict__ camlist,
                                                        const LinCam* __restrict__ cams, SrcTable src,
                                                        uint8_t* __restrict__ out, uint32_t ostep, int w, int h)
 {
+    constexpr int R = kLinRows;
     const LinTile J = ldc(tiles + blockIdx.x);
-    const int X = J.X + (threadIdx.x & 63), Y = J.Y + (threadIdx.x >> 6);
-    if (X >= w || Y >= h) return;
-    float a0 = 0.f, a1 = 0.f, a2 = 0.f;
+    const int X = J.X + (threadIdx.x & 63), Y0 = J.Y + (threadIdx.x >> 6);
+    if (X >= w || Y0 >= h) return;
+    float a[R][3];
+#pragma unroll
+    for (int r = 0; r < R; r++) a[r][0] = a[r][1] = a[r][2] = 0.f;
     for (int k = 0; k < J.cnt; k++) {
         const LinCam C = ldc(cams + ldc(camlist + J.off + k));
-        const size_t q = (size_t)Y * C.pitch + X;
-        const float wv = ldg(C.w + q);
-        if (wv == 0.f) continue;   // adds nothing
-        const float2 m = ldg_f2(C.map + q);
-        if (!(m.x >= 0.f && m.y >= 0.f && m.x < (float)C.sw && m.y < (float)C.sh)) continue;
         const uint8_t* sp = src.p[C.src];
-        uint32_t o[4], p[4];
-        tap_offsets_fast(src.step[C.src], C.sw, C.sh, m.x, m.y, o);
+        const uint32_t sstep = src.step[C.src];
+        float wv[R];
+        float2 m[R];
 #pragma unroll
-        for (int t = 0; t < 4; t++) p[t] = ldg((const uint32_t*)(sp + o[t]));
-        bool tie = false;
-        uint32_t v = bilinear_fast(__builtin_amdgcn_fractf(m.x), __builtin_amdgcn_fractf(m.y), p, tie);
-        if (tie) v = bilinear_combine((double)m.x, (double)m.y, p, nullptr);
-        a0 = __fadd_rn(a0, __fmul_rn((float)(v & 255u), wv));
-        a1 = __fadd_rn(a1, __fmul_rn((float)((v >> 8) & 255u), wv));
-        a2 = __fadd_rn(a2, __fmul_rn((float)((v >> 16) & 255u), wv));
+        for (int r = 0; r < R; r++) {
+            const int Y = Y0 + 4 * r;
+            wv[r] = Y < h ? ldg(C.w + (size_t)Y * C.pitch + X) : 0.f;
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+            m[r] = wv[r] != 0.f ? ldg_f2(C.map + (size_t)(Y0 + 4 * r) * C.pitch + X) : make_float2(-1.f, -1.f);
+        bool ok[R];
+        uint32_t p[R][4];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            // a zero weight adds nothing; an invalid map point adds nothing
+            ok[r] = m[r].x >= 0.f && m[r].y >= 0.f && m[r].x < (float)C.sw && m[r].y < (float)C.sh;
+            if (ok[r]) {
+                uint32_t o[4];
+                tap_offsets_fast(sstep, C.sw, C.sh, m[r].x, m[r].y, o);
+#pragma unroll
+                for (int t = 0; t < 4; t++) p[r][t] = ldg((const uint32_t*)(sp + o[t]));
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (!ok[r]) continue;
+            bool tie = false;
+            uint32_t v = bilinear_fast(__builtin_amdgcn_fractf(m[r].x), __builtin_amdgcn_fractf(m[r].y), p[r], tie);
+            if (tie) v = bilinear_combine((double)m[r].x, (double)m[r].y, p[r], nullptr);
+            a[r][0] = __fadd_rn(a[r][0], __fmul_rn((float)(v & 255u), wv[r]));
+            a[r][1] = __fadd_rn(a[r][1], __fmul_rn((float)((v >> 8) & 255u), wv[r]));
+            a[r][2] = __fadd_rn(a[r][2], __fmul_rn((float)((v >> 16) & 255u), wv[r]));
+        }
     }
-    uint32_t o = __builtin_amdgcn_cvt_pk_u8_f32(a0, 0u, 0u);
-    o = __builtin_amdgcn_cvt_pk_u8_f32(a1, 1u, o);
-    o = __builtin_amdgcn_cvt_pk_u8_f32(a2, 2u, o);
-    stg((uint32_t*)(out + (size_t)Y * ostep) + X, o);
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int Y = Y0 + 4 * r;
+        if (Y >= h) break;
+        uint32_t o = __builtin_amdgcn_cvt_pk_u8_f32(a[r][0], 0u, 0u);
+        o = __builtin_amdgcn_cvt_pk_u8_f32(a[r][1], 1u, o);
+        o = __builtin_amdgcn_cvt_pk_u8_f32(a[r][2], 2u, o);
+        stg((uint32_t*)(out + (size_t)Y * ostep) + X, o);
+    }
 }
 
 __global__ __launch_bounds__(256) void k_linear_blend32s(SrcTable images, int n, const int32_t* __restrict__ wts,

@@ -163,7 +163,7 @@ struct SrcTable {
 };
 
 // Linear-blend render: a camera's full-canvas map and weight planes (pitch in
-// elements) and its frame (SrcTable index, size); a canvas tile of 64 x 4
+// elements) and its frame (SrcTable index, size); a canvas tile of 64 x kLinTH
 // pixels with its cameras camlist[off .. off + cnt).
 struct LinCam {
     const float2* map;
@@ -171,7 +171,8 @@ struct LinCam {
     int pitch, src, sw, sh;
 };
 struct LinTile { int X, Y, off, cnt; };
-constexpr int kLinTW = 64, kLinTH = 4;
+constexpr int kLinRows = 4;                       // canvas rows per thread (4 apart)
+constexpr int kLinTW = 64, kLinTH = 4 * kLinRows;  // a 256-thread block
 
 constexpr int kMaxOut = 32;     // output canvases addressable by one launch
 // Output formats of the level-0 kernel: the BGRx canvas, or the encoder's
