@@ -42,6 +42,8 @@ extern "C" {
 #define PANO_ERR_NOMEM (-4)       /* device or host allocation failed           */
 #define PANO_ERR_UNSUPPORTED (-5) /* valid request this build does not handle   */
 #define PANO_ERR_BUDGET (-6)      /* admission refused: device memory budget    */
+#define PANO_ERR_IO (-7)          /* a file could not be opened or written      */
+#define PANO_ERR_FORMAT (-8)      /* a rig file field is not a number           */
 
 /* ---- image type codes (OpenCV CV_MAKETYPE values) ---------------------- */
 #define PANO_8UC1 0
@@ -76,6 +78,28 @@ typedef struct pano_photo_param {
     double hfov, vfov, alpha, beta, gamma, shiftX, shiftY, shearX, shearY;
     double yaw, pitch, roll;
 } pano_photo_param;
+
+/* ---- rig files: PhotoParam XML / PTGui .pts (Warp/PhotoParam.cpp) --------
+ * Loaders write up to cap params to out and return how many the file holds
+ * (0 when the file is missing, is not one of the layouts or holds no camera:
+ * the reference's empty result / false), or a negative status.
+ *
+ * loadPhotoParams (PhotoParam.cpp:756-770): extension "pts" -> PTS, any other
+ * -> XML; no extension -> PANO_ERR_ARG. */
+int pano_load_photo_params(const char* path, pano_photo_param* out, int cap);
+/* loadPhotoParamFromXML (PhotoParam.cpp:506-511): the per-camera layout
+ * (:273-504), else the shared-lens layout (:17-271).  An element whose text is
+ * not a number -> PANO_ERR_FORMAT (the reference throws there). */
+int pano_load_photo_params_xml(const char* path, pano_photo_param* out, int cap);
+/* loadPhotoParamFromPTS (PhotoParam.cpp:681-754). */
+int pano_load_photo_params_pts(const char* path, pano_photo_param* out, int cap);
+/* exportPhotoParamToXML (PhotoParam.cpp:772-925): the per-camera layout. */
+int pano_export_photo_params_xml(const char* path, const pano_photo_param* params, int num);
+/* rotateCameras (PhotoParam.cpp:931-967): each camera's pose pre-multiplied by
+ * the rotation (yaw, pitch, roll), given in radians. */
+int pano_rotate_cameras(pano_photo_param* params, int num, double yaw, double pitch, double roll);
+/* rotatePhotoParamInXML (PhotoParam.cpp:969-975). */
+int pano_rotate_photo_params_xml(const char* src, const char* dst, double yaw, double pitch, double roll);
 
 /* ---- multiband blender: CudaTilingMultibandBlendFast -------------------
  * Replaces the class in CudaAccel/CudaInterface.h:45-82 (implementation

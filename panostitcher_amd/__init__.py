@@ -30,6 +30,8 @@ LIB_PATH = os.path.join(_HERE, "libpanohip.so")
 
 PANO_OK = 0
 PANO_ERR_BUDGET = -6
+PANO_ERR_IO = -7
+PANO_ERR_FORMAT = -8
 PANO_8UC4 = 24
 PANO_16SC4 = 27
 PANO_32FC4 = 29
@@ -104,6 +106,12 @@ def lib():
             "pano_get_lut": ([vp, C.c_double], i),
             "pano_exposure_luts": ([i, vp, vp, vp, vp], i),
             "pano_generate_map": ([vp, i, i, i, i, vp, vp, vp], i),
+            "pano_load_photo_params": ([C.c_char_p, vp, i], i),
+            "pano_load_photo_params_xml": ([C.c_char_p, vp, i], i),
+            "pano_load_photo_params_pts": ([C.c_char_p, vp, i], i),
+            "pano_export_photo_params_xml": ([C.c_char_p, vp, i], i),
+            "pano_rotate_cameras": ([vp, i, C.c_double, C.c_double, C.c_double], i),
+            "pano_rotate_photo_params_xml": ([C.c_char_p, C.c_char_p, C.c_double, C.c_double, C.c_double], i),
             "pano_render_create": ([pp], i),
             "pano_render_destroy": ([vp], None),
             "pano_render_prepare": ([vp, i, vp, vp, vp, i, i, i], i),
@@ -185,6 +193,8 @@ def exported_symbols():
         "pano_linear_render_prepare_maps", "pano_linear_render_render", "pano_linear_render_weights",
         "pano_nonintersecting_masks", "pano_render_prepare_ms", "pano_generate_map_device",
         "pano_reproject_weighted_accumulate", "pano_reproject_params",
+        "pano_load_photo_params", "pano_load_photo_params_xml", "pano_load_photo_params_pts",
+        "pano_export_photo_params_xml", "pano_rotate_cameras", "pano_rotate_photo_params_xml",
     ]
 
 
@@ -266,6 +276,65 @@ class PhotoParam(C.Structure):
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+# ---------------------------------------------------------------- rig files --
+
+def _load_params(fn, path, what):
+    f = getattr(lib(), fn)
+    p = os.fsencode(path)
+    n = f(p, None, 0)
+    if n < 0:
+        _check(n, what)
+    if n == 0:
+        return []
+    arr = (PhotoParam * n)()
+    m = f(p, arr, n)
+    if m < 0:
+        _check(m, what)
+    return list(arr[:min(m, n)])
+
+
+def load_photo_params(path) -> list:
+    """loadPhotoParams (Warp/PhotoParam.cpp:756-770): ".pts" files as PTGui
+    projects, others as XML; an empty list where the reference returns false.
+    A path without an extension raises, as the reference reports an error."""
+    return _load_params("pano_load_photo_params", path, "loadPhotoParams")
+
+
+def load_photo_param_from_xml(path) -> list:
+    """loadPhotoParamFromXML (PhotoParam.cpp:506-511); a non-numeric field raises."""
+    return _load_params("pano_load_photo_params_xml", path, "loadPhotoParamFromXML")
+
+
+def load_photo_param_from_pts(path) -> list:
+    """loadPhotoParamFromPTS (PhotoParam.cpp:681-754)."""
+    return _load_params("pano_load_photo_params_pts", path, "loadPhotoParamFromPTS")
+
+
+def export_photo_param_to_xml(path, params) -> None:
+    """exportPhotoParamToXML (PhotoParam.cpp:772-925)."""
+    arr = (PhotoParam * len(params))(*params)
+    _check(lib().pano_export_photo_params_xml(os.fsencode(path), arr, len(params)), "exportPhotoParamToXML")
+
+
+def rotate_cameras(params, yaw: float, pitch: float, roll: float) -> None:
+    """rotateCameras (PhotoParam.cpp:962-967), in place; the rotation in radians."""
+    arr = (PhotoParam * len(params))(*params)
+    _check(lib().pano_rotate_cameras(arr, len(params), yaw, pitch, roll), "rotateCameras")
+    for k, q in enumerate(arr):
+        C.memmove(C.addressof(params[k]), C.addressof(q), C.sizeof(PhotoParam))
+
+
+def rotate_camera(param: PhotoParam, yaw: float, pitch: float, roll: float) -> None:
+    """rotateCamera (PhotoParam.cpp:931-960), in place; the rotation in radians."""
+    _check(lib().pano_rotate_cameras(C.byref(param), 1, yaw, pitch, roll), "rotateCamera")
+
+
+def rotate_photo_param_in_xml(src, dst, yaw: float, pitch: float, roll: float) -> None:
+    """rotatePhotoParamInXML (PhotoParam.cpp:969-975)."""
+    _check(lib().pano_rotate_photo_params_xml(os.fsencode(src), os.fsencode(dst), yaw, pitch, roll),
+           "rotatePhotoParamInXML")
 
 
 def get_lut(k: float) -> np.ndarray:
@@ -755,7 +824,16 @@ class PanoramaRender:
             self._h = None
 
     def prepare(self, params, src_sizes, dst_size, max_levels: int) -> bool:
+        """params: PhotoParams, or a rig file path loaded by loadPhotoParams as
+        CPUPanoramaRender::prepare does (RicohUtil.cpp:1437-1443: False when it
+        yields no camera).  src_sizes: one (w, h) per camera, or one for all."""
+        if isinstance(params, (str, bytes, os.PathLike)):
+            params = load_photo_params(params)
+            if not params:
+                return False
         n = len(params)
+        if len(src_sizes) == 2 and all(isinstance(v, (int, np.integer)) for v in src_sizes):
+            src_sizes = [tuple(src_sizes)] * n
         arr = (PhotoParam * n)(*params)
         sw = np.array([s[0] for s in src_sizes], np.int32)
         shh = np.array([s[1] for s in src_sizes], np.int32)

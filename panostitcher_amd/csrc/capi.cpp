@@ -903,6 +903,8 @@ const char* pano_status_string(int status)
     case PANO_ERR_NOMEM: return "out of memory";
     case PANO_ERR_UNSUPPORTED: return "unsupported";
     case PANO_ERR_BUDGET: return "device memory budget exceeded";
+    case PANO_ERR_IO: return "file could not be opened or written";
+    case PANO_ERR_FORMAT: return "rig file field is not a number";
     default: return "unknown status";
     }
 }

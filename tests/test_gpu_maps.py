@@ -95,3 +95,30 @@ def test_weighted_accumulate_matches_linear_render():
     got = np.clip(np.rint(a[..., :3]), 0, 255).astype(np.uint8)   # numpy rint: half to even, as convertTo
     np.testing.assert_array_equal(got, ref)
     assert not a[..., 3].any()
+
+
+def test_render_prepared_from_rig_file():
+    # CPUPanoramaRender::prepare(path, ...) loads the rig with loadPhotoParams;
+    # the PTGui fixture holds three cameras (two fisheyes, one rectilinear)
+    import os
+
+    path = os.path.join(os.path.dirname(__file__), "golden", "rigs", "ptgui.pts")
+    params = P.load_photo_params(path)
+    assert len(params) == 3
+    sw, sh, dw, dh, levels = 1920, 1920, 1024, 512, 5
+    rng = np.random.default_rng(11)
+    srcs = [rng.integers(0, 256, (sh, sw, 3), dtype=np.uint8) for _ in params]
+    r = P.PanoramaRender()
+    note("render prepared from ptgui.pts 3 x 1920x1920 -> 1024x512")
+    assert r.prepare(path, (sw, sh), (dw, dh), levels)
+    assert not P.PanoramaRender().prepare(path + ".missing", (sw, sh), (dw, dh), levels)
+    maps = [P.generate_map_device(q, sw, sh, dw, dh) for q in params]
+    xs = [from_dev(m[0]) for m in maps]
+    ys = [from_dev(m[1]) for m in maps]
+    ms = [from_dev(m[2]) for m in maps]
+    ob = O.Blender()
+    assert ob.prepare(np.stack(ms), levels)
+    ref = ob.render(srcs, xs, ys, None, nthreads=8)
+    out = torch.zeros((dh, dw, 4), dtype=torch.uint8, device="cuda")
+    assert r.render([to_dev(synth.bgrx(s)) for s in srcs], out)
+    assert_parity(from_dev(out)[..., :3], ref, exact=True, label="render from rig file")
