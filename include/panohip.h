@@ -340,10 +340,14 @@ int pano_render_render(pano_render* r, int num, const uint8_t* const* srcs,
 int pano_render_render_to(pano_render* r, int num, const uint8_t* const* srcs, const size_t* src_steps,
                           const uint8_t* luts, const pano_output* out, hipStream_t stream);
 
-/* Host wall time of the last pano_render_prepare_maps, in ms: map support
- * scan, seams (on the device), host tables (weights, work lists), device
- * build (uploads, fused maps). */
+/* Wall time of the last prepare, in ms: maps (generation or upload) and their
+ * check, seams (on the device), tables (boxes, weight pyramids, weight sums,
+ * owners, work lists), build (device residency, fused map storage). */
 int pano_render_prepare_ms(const pano_render* r, double* ms4);
+/* Where the fused 8-bit render's prepare builds its tables, weight pyramids
+ * and map storage: on the device (on != 0, the default, up to 32 cameras) or
+ * on the host (the reference prepares on the host; both give the same rig). */
+int pano_render_set_prepare_on_device(pano_render* r, int on);
 int pano_render_num_images(const pano_render* r);
 int pano_render_num_levels(const pano_render* r);
 int64_t pano_render_mem_bytes(const pano_render* r);

@@ -83,6 +83,7 @@ def lib():
             "pano_linear_weights": ([i, i, i, pp, psz, i, i, pp, psz], i),
             "pano_nonintersecting_masks": ([i, i, i, pp, psz, pp, psz], i),
             "pano_render_prepare_ms": ([vp, vp], i),
+            "pano_render_set_prepare_on_device": ([vp, i], i),
             "pano_generate_map_device": ([vp, i, i, i, i, vp, vp, sz, vp, sz, vp], i),
             "pano_reproject_weighted_accumulate": ([vp, sz, i, i, vp, vp, sz, vp, sz, vp, sz, i, i, vp], i),
             "pano_reproject_params": ([vp, sz, i, i, vp, i, i, i, vp, sz, vp, sz, vp], i),
@@ -195,6 +196,7 @@ def exported_symbols():
         "pano_reproject_weighted_accumulate", "pano_reproject_params",
         "pano_load_photo_params", "pano_load_photo_params_xml", "pano_load_photo_params_pts",
         "pano_export_photo_params_xml", "pano_rotate_cameras", "pano_rotate_photo_params_xml",
+        "pano_render_set_prepare_on_device",
     ]
 
 
@@ -902,8 +904,12 @@ class PanoramaRender:
         _check(st, "render")
         return True
 
+    def set_prepare_on_device(self, on: bool = True) -> None:
+        """Build the fused render's tables on the device (default) or the host."""
+        _check(lib().pano_render_set_prepare_on_device(self._h, int(bool(on))), "set_prepare_on_device")
+
     def prepare_ms(self):
-        """Host wall time of the last prepare_maps by phase (ms)."""
+        """Wall time of the last prepare by phase (ms)."""
         ms = np.zeros(4, np.float64)
         _check(lib().pano_render_prepare_ms(self._h, C.c_void_p(ms.ctypes.data)), "prepare_ms")
         return dict(zip(("support", "seams", "tables", "build"), ms.round(1).tolist()))

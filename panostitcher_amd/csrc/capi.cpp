@@ -285,6 +285,11 @@ struct DeviceRig {
     unsigned long long* d_gstats = nullptr;
     double* d_gains = nullptr;
     size_t model_bytes[8] = {};
+    // prepare on the device: tables and the device maps / masks the build reads
+    // instead of the host ones (n planes, pitch prep.cols)
+    const DevTables* dev = nullptr;
+    const float* dev_map = nullptr;         // camera i: x plane at 2 i P, y plane at (2 i + 1) P
+    const uint8_t* dev_masks = nullptr;
 
     int levels() const { return prep.levels; }
 
@@ -308,7 +313,8 @@ struct DeviceRig {
                 if (j > 0) pl.add(g.pixels() * 2);
             }
         if (fused) pl.add(P * 4);
-        const bool gain = fused && xmaps && n <= kGainMaxCams;
+        const bool have_maps = xmaps || dev_map;
+        const bool gain = fused && have_maps && n <= kGainMaxCams;
         if (gain) {
             pl.add(P);
             pl.add(sizeof(unsigned long long) * n * n * 5);
@@ -334,7 +340,7 @@ struct DeviceRig {
         pl.add(sizeof(TileJob) * prep.fin.size());
         pl.add(P);  // label
         if (keep_masks) pl.add(P * n);
-        if (xmaps) {
+        if (have_maps) {
             pl.add(sizeof(RemapJob) * prep.remap.size());
             pl.add(sizeof(RemapCam) * n);
             for (int i = 0; i < n; i++) {
@@ -366,8 +372,14 @@ struct DeviceRig {
                 }
                 if (j > 0) {
                     p.w = (uint16_t*)arena.take(g.pixels() * 2);
-                    const std::vector<uint16_t>& w = prep.weight[(size_t)i * (L + 1) + j];
-                    if (!w.empty()) HIP_TRY(hipMemcpy(p.w, w.data(), w.size() * 2, hipMemcpyHostToDevice));
+                    if (dev) {
+                        if (g.pixels())
+                            HIP_TRY(hipMemcpy(p.w, dev->w + dev->woff[(size_t)i * (L + 1) + j], g.pixels() * 2,
+                                              hipMemcpyDeviceToDevice));
+                    } else {
+                        const std::vector<uint16_t>& w = prep.weight[(size_t)i * (L + 1) + j];
+                        if (!w.empty()) HIP_TRY(hipMemcpy(p.w, w.data(), w.size() * 2, hipMemcpyHostToDevice));
+                    }
                 }
             }
         std::vector<CanvasLevel> canvas(L + 1);
@@ -380,14 +392,18 @@ struct DeviceRig {
             else c.c = (int4*)arena.take((size_t)c.cw * c.ch * 16);
             if (!prep.full) {
                 c.wsum = (uint16_t*)arena.take((size_t)c.cw * c.ch * 2);
-                HIP_TRY(hipMemcpy(c.wsum, prep.wsum[j].data(), prep.wsum[j].size() * 2,
-                                  hipMemcpyHostToDevice));
+                if (dev)
+                    HIP_TRY(hipMemcpy(c.wsum, dev->wsum[j], (size_t)c.cw * c.ch * 2, hipMemcpyDeviceToDevice));
+                else
+                    HIP_TRY(hipMemcpy(c.wsum, prep.wsum[j].data(), prep.wsum[j].size() * 2,
+                                      hipMemcpyHostToDevice));
             }
         }
         canvas[0].cw = prep.cols;
         canvas[0].ch = prep.rows;
         d_label = (uint8_t*)arena.take(P);
-        HIP_TRY(hipMemcpy(d_label, prep.label.data(), P, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_label, dev ? dev->label : prep.label.data(), P,
+                          dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice));
         if (keep_masks) {
             d_masks = (uint8_t*)arena.take(P * n);
             for (int i = 0; i < n; i++)
@@ -462,7 +478,7 @@ struct DeviceRig {
             HIP_TRY(hipMemset(d_g0comp, 0, P * 4));
         }
         if (err) return err;
-        if (xmaps) {
+        if (have_maps) {
             d_remap = upload_vec(arena, prep.remap, err);
             std::vector<RemapCam> cams(n);
             for (int i = 0; i < n; i++) {
@@ -480,6 +496,17 @@ struct DeviceRig {
                 float2* dm = (float2*)arena.take(cells * 8);
                 uint8_t* dk = fused ? nullptr : (uint8_t*)arena.take(cells);
                 if (!dm || (!fused && !dk && cells)) return PANO_ERR_NOMEM;
+                c.map = dm;
+                c.mask = dk;
+                c.mpitch = mp;
+                if (dev_map) {
+                    // fused storage packed on the device from the device maps
+                    if (!fused) return PANO_ERR_ARG;
+                    HIP_TRY(launch_pack_map(dev_map + 2 * P * i, dev_map + (2 * (size_t)i + 1) * P,
+                                            dev_masks + P * i, d_label, prep.cols, i, g, m, mp, src_w[i], src_h[i],
+                                            dm, nullptr));
+                    continue;
+                }
                 std::vector<float2> hm(cells, fused ? fused_point(-1.f, -1.f, 1, 1, 0) : make_float2(-1.f, -1.f));
                 std::vector<uint8_t> hk(fused ? 0 : cells, 0);
                 // entry (sy, sx) holds canvas (x0 - m + sx, y0 - m + sy) after border
@@ -541,15 +568,21 @@ struct DeviceRig {
             if (err) return err;
         }
         if (gain) {
-            std::vector<uint8_t> cover(P, 0);
-            for (int i = 0; i < n; i++)
-                for (int y = 0; y < prep.rows; y++) {
-                    const uint8_t* m = masks[i] + (size_t)y * mstep[i];
-                    uint8_t* c = cover.data() + (size_t)y * prep.cols;
-                    for (int x = 0; x < prep.cols; x++)
-                        if (m[x]) c[x] |= (uint8_t)(1u << i);
-                }
-            d_cover = upload_vec(arena, cover, err);
+            if (dev) {
+                d_cover = (uint8_t*)arena.take(P);
+                if (!d_cover || !dev->cover) return PANO_ERR_NOMEM;
+                HIP_TRY(hipMemcpy(d_cover, dev->cover, P, hipMemcpyDeviceToDevice));
+            } else {
+                std::vector<uint8_t> cover(P, 0);
+                for (int i = 0; i < n; i++)
+                    for (int y = 0; y < prep.rows; y++) {
+                        const uint8_t* m = masks[i] + (size_t)y * mstep[i];
+                        uint8_t* c = cover.data() + (size_t)y * prep.cols;
+                        for (int x = 0; x < prep.cols; x++)
+                            if (m[x]) c[x] |= (uint8_t)(1u << i);
+                    }
+                d_cover = upload_vec(arena, cover, err);
+            }
             d_gstats = (unsigned long long*)arena.take(sizeof(unsigned long long) * n * n * 5);
             d_gains = (double*)arena.take(sizeof(double) * (3 * n + 1));
             if (!d_gstats || !d_gains) return PANO_ERR_NOMEM;
@@ -614,7 +647,7 @@ struct DeviceRig {
             for (size_t k = 0; k < ls.size(); k++) ls[k].segs = d_segs + k;
             seq = ls;
         }
-        compute_model_bytes(xmaps != nullptr, src_w, src_h);
+        compute_model_bytes(have_maps, src_w, src_h);
         return err;
     }
 
@@ -779,7 +812,8 @@ struct pano_render {
     int lut_state = 0;                // the device's "apply LUTs" byte
     std::vector<uint8_t> lut_block;   // n x 768 LUTs + the flag byte
     uint8_t* bgrx_scratch = nullptr;  // 16-bit path with encoder-plane output: the BGRx canvas
-    double prep_ms[4] = {};           // last prepare: map support, seams, host tables, device build
+    double prep_ms[4] = {};           // last prepare: map support, seams, tables, device build
+    bool prepare_on_device = true;    // fused-path tables, map storage and seams built on the device
     ~pano_render()
     {
         if (bgrx_scratch) (void)hipFree(bgrx_scratch);
@@ -1291,6 +1325,105 @@ int pano_render_create(pano_render** out)
 
 void pano_render_destroy(pano_render* r) { delete r; }
 
+}  // extern "C"
+
+namespace {
+
+// Device buffers of one device prepare: maps (camera i: x plane at 2 i P, y
+// plane at (2 i + 1) P), masks and unique masks (n planes of P bytes).
+struct PrepBufs {
+    float* maps = nullptr;
+    uint8_t* masks = nullptr;
+    uint8_t* unique = nullptr;
+    ~PrepBufs()
+    {
+        if (maps) (void)hipFree(maps);
+        if (masks) (void)hipFree(masks);
+        if (unique) (void)hipFree(unique);
+    }
+    int alloc(int n, size_t P)
+    {
+        if (hipMalloc((void**)&maps, P * 8 * n) != hipSuccess || hipMalloc((void**)&masks, P * n) != hipSuccess ||
+            hipMalloc((void**)&unique, P * n) != hipSuccess)
+            return PANO_ERR_NOMEM;
+        return PANO_OK;
+    }
+};
+
+using prep_clock = std::chrono::steady_clock;
+
+double ms_between(prep_clock::time_point a, prep_clock::time_point b)
+{
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// The fused render's prepare with seams, tables and map storage built on the
+// device from consistent device maps and masks.  t0: start of the prepare
+// (map generation or checks included in the first timing).
+int render_prepare_device(pano_render* r, int num, const int* src_w, const int* src_h, int dst_w, int dst_h,
+                          PrepBufs& b, int max_levels, prep_clock::time_point t0)
+{
+    const size_t P = (size_t)dst_w * dst_h;
+    const auto t1 = prep_clock::now();
+    HIP_TRY(hipMemcpy(b.unique, b.masks, P * num, hipMemcpyDeviceToDevice));
+    HIP_TRY(device_seams_inplace(num, dst_h, dst_w, b.unique));
+    const auto t2 = prep_clock::now();
+    r->rig.reset(new (std::nothrow) DeviceRig());
+    if (!r->rig) return PANO_ERR_NOMEM;
+    r->rig->kind = kPixU8;
+    DevTables dt;
+    int err = prepare_tables_device(num, dst_h, dst_w, b.masks, b.unique, max_levels, true, r->rig->prep, dt);
+    if (err) return err;
+    const auto t3 = prep_clock::now();
+    std::vector<size_t> steps(num, (size_t)dst_w);
+    r->rig->dev = &dt;
+    r->rig->dev_map = b.maps;
+    r->rig->dev_masks = b.masks;
+    err = r->rig->build(true, false, nullptr, nullptr, nullptr, steps.data(), src_w, src_h);
+    r->rig->dev = nullptr;
+    r->rig->dev_map = nullptr;
+    r->rig->dev_masks = nullptr;
+    if (err) return err;
+    HIP_TRY(hipDeviceSynchronize());
+    const auto t4 = prep_clock::now();
+    r->prep_ms[0] = ms_between(t0, t1);
+    r->prep_ms[1] = ms_between(t1, t2);
+    r->prep_ms[2] = ms_between(t2, t3);
+    r->prep_ms[3] = ms_between(t3, t4);
+    err = r->ring.init((size_t)num * 768 + 1);
+    if (err) return err;
+    r->src_w.assign(src_w, src_w + num);
+    r->src_h.assign(src_h, src_h + num);
+    r->dst_w = dst_w;
+    r->dst_h = dst_h;
+    r->ok = true;
+    return PANO_OK;
+}
+
+// Size limits of the render path (shared by both prepares).
+int render_prepare_limits(int num, const int* src_w, const int* src_h, int dst_w, int dst_h)
+{
+    if ((size_t)dst_w * dst_h * 4 > 0xffffffffu) return PANO_ERR_UNSUPPORTED;   // 32-bit canvas offsets
+    for (int i = 0; i < num; i++) {
+        if (src_w[i] <= 0 || src_h[i] <= 0) return PANO_ERR_ARG;
+        // 24-bit row offsets in the remap, 32-bit frame offsets
+        if (lut_step(src_w[i]) >= (1u << 24) || lut_step(src_w[i]) * (uint64_t)src_h[i] > 0xffffffffu)
+            return PANO_ERR_UNSUPPORTED;
+    }
+    return PANO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pano_render_set_prepare_on_device(pano_render* r, int on)
+{
+    if (!r) return PANO_ERR_ARG;
+    r->prepare_on_device = on != 0;
+    return PANO_OK;
+}
+
 int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const int* src_h,
                              int dst_w, int dst_h, const float* const* xmaps,
                              const float* const* ymaps, const uint8_t* const* masks,
@@ -1328,12 +1461,23 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
         }
         if (!ok) consistent = false;
     });
-    r->rig.reset(new (std::nothrow) DeviceRig());
-    if (!r->rig) return PANO_ERR_NOMEM;
     // 8-bit planes hold every Gaussian level exactly when sampled pixels lie
     // inside the masks (as getReprojectMapAndMask produces them); otherwise the
     // reference's pyramid can leave [0, 255] and 16-bit planes are used.
     const bool fused = consistent && true_num_levels(dst_w, dst_h, max_levels, 2) >= 1;
+    if (fused && r->prepare_on_device && num <= 32) {
+        PrepBufs b;
+        int e = b.alloc(num, P);
+        if (e) return e;
+        for (int i = 0; i < num; i++) {
+            HIP_TRY(hipMemcpy(b.maps + 2 * P * i, xmaps[i], P * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(b.maps + (2 * (size_t)i + 1) * P, ymaps[i], P * 4, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(b.masks + P * i, masks[i], P, hipMemcpyHostToDevice));
+        }
+        return render_prepare_device(r, num, src_w, src_h, dst_w, dst_h, b, max_levels, t0);
+    }
+    r->rig.reset(new (std::nothrow) DeviceRig());
+    if (!r->rig) return PANO_ERR_NOMEM;
     r->rig->kind = fused ? kPixU8 : kPixS16;
     const auto t1 = clock::now();
     std::vector<uint8_t> unique;
@@ -1378,6 +1522,33 @@ int pano_render_prepare(pano_render* r, int num, const pano_photo_param* params,
     if (!r || num <= 0 || !params || !src_w || !src_h) return PANO_ERR_ARG;
     if ((dst_w & 1) || (dst_h & 1) || dst_h * 2 != dst_w || dst_w <= 0) return PANO_ERR_ARG;
     const size_t P = (size_t)dst_w * dst_h;
+    if (r->prepare_on_device && num <= 32) {
+        if (max_levels <= 0) return PANO_ERR_ARG;
+        int e = render_prepare_limits(num, src_w, src_h, dst_w, dst_h);
+        if (e) return e;
+        r->ok = false;
+        const auto t0 = prep_clock::now();
+        PrepBufs b;
+        e = b.alloc(num, P);
+        if (e) return e;
+        for (int i = 0; i < num && !e; i++)
+            e = generate_map_device(params[i], src_w[i], src_h[i], dst_w, dst_h, b.maps + 2 * P * i,
+                                    b.maps + (2 * (size_t)i + 1) * P, (size_t)dst_w * 4, b.masks + P * i,
+                                    (size_t)dst_w, nullptr);
+        if (e) return e;
+        // masks made by the generator hold every valid map point; checked all the same
+        bool all = true;
+        for (int i = 0; i < num && !e; i++) {
+            bool c = true;
+            e = maps_consistent_device(b.maps + 2 * P * i, b.maps + (2 * (size_t)i + 1) * P, b.masks + P * i, P,
+                                       src_w[i], src_h[i], c);
+            all = all && c;
+        }
+        if (e) return e;
+        if (all && true_num_levels(dst_w, dst_h, max_levels, 2) >= 1)
+            return render_prepare_device(r, num, src_w, src_h, dst_w, dst_h, b, max_levels, t0);
+        // otherwise the host prepare below, on maps read back
+    }
     std::vector<std::vector<float>> xm(num, std::vector<float>(P)), ym(num, std::vector<float>(P));
     std::vector<std::vector<uint8_t>> mk(num, std::vector<uint8_t>(P));
     std::vector<const float*> xp(num), yp(num);

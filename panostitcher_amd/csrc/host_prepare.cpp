@@ -154,6 +154,135 @@ int true_num_levels(int width, int height, int max_levels, int min_length)
     return std::min(num, std::min(lw, lh));
 }
 
+void camera_boxes(int L, const int* lw, const int* lh, std::vector<uint8_t> colf, std::vector<uint8_t> rowf,
+                  Geom* geom)
+{
+    for (int j = 0; j <= L; j++) {
+        if (j > 0) {
+            std::vector<uint8_t> c2(lw[j], 0), r2(lh[j], 0);
+            for (int q = 0; q < lw[j]; q++)
+                for (int b = -2; b <= 2 && !c2[q]; b++) c2[q] = colf[wrap_idx(2 * q + b, lw[j - 1])];
+            for (int q = 0; q < lh[j]; q++)
+                for (int a = -2; a <= 2 && !r2[q]; a++) r2[q] = rowf[reflect_idx(2 * q + a, lh[j - 1])];
+            colf.swap(c2);
+            rowf.swap(r2);
+        }
+        Geom& g = geom[j];
+        g.cw = lw[j];
+        g.ch = lh[j];
+        int y0 = -1, y1 = -1;
+        for (int y = 0; y < g.ch; y++)
+            if (rowf[y]) {
+                if (y0 < 0) y0 = y;
+                y1 = y;
+            }
+        int x0, len;
+        cyclic_interval(colf, g.cw, x0, len);
+        if (y0 < 0 || len == 0) {
+            g.x0 = g.y0 = g.bw = g.bh = 0;
+        } else {
+            g.x0 = x0;
+            g.bw = len;
+            g.y0 = y0;
+            g.bh = y1 - y0 + 1;
+        }
+        g.pitch = round_pitch(g.bw);
+    }
+}
+
+void work_lists(Prepared& out, const TileValidFn& tile_has_valid, const CamHitsFn& cam_hits,
+                const TileOwnersFn& tile_owners)
+{
+    const int n = out.n, L = out.levels, rows = out.rows, cols = out.cols;
+    const bool u8 = out.kind == kPixU8, fused0 = out.fused0;
+    const int dtw = u8 ? kFTW : kDownTW, dth = u8 ? kFTH : kDownTH;
+    const int ttw = u8 ? 2 * kQTW : kTileW, tth = u8 ? 2 * kQTH : kTileH;
+    out.down.assign(L, std::vector<DownJob>());
+    for (int j = 0; j < L; j++) {
+        for (int i = 0; i < n; i++) {
+            const Geom& gd = out.geom[(size_t)i * (L + 1) + j + 1];
+            if (gd.empty()) continue;
+            int segs[2][2] = {{gd.x0, std::min(gd.x0 + gd.bw, gd.cw)}, {0, gd.x0 + gd.bw - gd.cw}};
+            for (int sgi = 0; sgi < 2; sgi++) {
+                int a = segs[sgi][0], b = segs[sgi][1];
+                for (int X = a; X < b; X += dtw)
+                    for (int Y = gd.y0; Y < gd.y0 + gd.bh; Y += dth) {
+                        DownJob d{};
+                        d.src = (fused0 && j == 0) ? i : i * (L + 1) + j;
+                        d.dst = i * (L + 1) + j + 1;
+                        d.X = X;
+                        d.Y = Y;
+                        d.nx = std::min(dtw, b - X);
+                        d.ny = std::min(dth, gd.y0 + gd.bh - Y);
+                        // outputs whose alpha is zero stay zero: skip empty tiles
+                        if (!tile_has_valid(i, j + 1, d.X, d.Y, d.nx, d.ny)) continue;
+                        out.down[j].push_back(d);
+                    }
+            }
+        }
+    }
+    out.blend.assign(L + 1, std::vector<TileJob>());
+    out.camlist.clear();
+    out.max_tile_cams = 0;
+    for (int j = 1; j <= L; j++) {
+        int tw = (out.lw[j] + ttw - 1) / ttw, th = (out.lh[j] + tth - 1) / tth;
+        std::vector<std::vector<int>> cams((size_t)tw * th);
+        for (int i = 0; i < n; i++) {
+            std::vector<uint8_t> hit((size_t)tw * th, 0);
+            cam_hits(j, i, hit);
+            for (size_t t = 0; t < hit.size(); t++)
+                if (hit[t]) cams[t].push_back(i * (L + 1) + j);
+        }
+        for (int ty = 0; ty < th; ty++)
+            for (int tx = 0; tx < tw; tx++) {
+                TileJob t{};
+                t.X = tx * ttw;
+                t.Y = ty * tth;
+                t.nx = std::min(ttw, out.lw[j] - t.X);
+                t.ny = std::min(tth, out.lh[j] - t.Y);
+                t.r = 0;
+                const std::vector<int>& c = cams[(size_t)ty * tw + tx];
+                t.off = (int)out.camlist.size();
+                t.cnt = (int)c.size();
+                t.flags = j == L ? kTileTop : 0;
+                out.camlist.insert(out.camlist.end(), c.begin(), c.end());
+                out.max_tile_cams = std::max(out.max_tile_cams, t.cnt);
+                out.blend[j].push_back(t);
+            }
+    }
+    {
+        int tw = (cols + ttw - 1) / ttw, th = (rows + tth - 1) / tth;
+        out.fin.clear();
+        out.fin_owners.clear();
+        for (int ty = 0; ty < th; ty++)
+            for (int tx = 0; tx < tw; tx++) {
+                TileJob t{};
+                t.X = tx * ttw;
+                t.Y = ty * tth;
+                t.nx = std::min(ttw, cols - t.X);
+                t.ny = std::min(tth, rows - t.Y);
+                t.off = (int)out.fin_owners.size();
+                if (u8) tile_owners(t, out.fin_owners);
+                t.cnt = (int)out.fin_owners.size() - t.off;
+                out.fin.push_back(t);
+            }
+    }
+    out.remap.clear();
+    for (int i = 0; i < n; i++) {
+        const Geom& g = out.geom[(size_t)i * (L + 1)];
+        for (int ly = 0; ly < g.bh; ly += kRemapTH)
+            for (int lx = 0; lx < g.bw; lx += kRemapTW) {
+                RemapJob r{};
+                r.cam = i;
+                r.lx = lx;
+                r.ly = ly;
+                r.nx = std::min(kRemapTW, g.bw - lx);
+                r.ny = std::min(kRemapTH, g.bh - ly);
+                out.remap.push_back(r);
+            }
+    }
+}
+
 int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps,
                 const uint8_t* const* support0, int max_levels, int min_length, bool full_boxes,
                 int kind, bool fused0, Prepared& out, const uint8_t* unique)
@@ -224,39 +353,8 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
                 rowf[y] = any;
             }
         }
-        for (int j = 0; j <= L; j++) {
-            if (j > 0) {
-                std::vector<uint8_t> c2(out.lw[j], 0), r2(out.lh[j], 0);
-                for (int q = 0; q < out.lw[j]; q++)
-                    for (int b = -2; b <= 2 && !c2[q]; b++)
-                        c2[q] = colf[wrap_idx(2 * q + b, out.lw[j - 1])];
-                for (int q = 0; q < out.lh[j]; q++)
-                    for (int a = -2; a <= 2 && !r2[q]; a++)
-                        r2[q] = rowf[reflect_idx(2 * q + a, out.lh[j - 1])];
-                colf.swap(c2);
-                rowf.swap(r2);
-            }
-            Geom& g = out.geom[(size_t)i * (L + 1) + j];
-            g.cw = out.lw[j];
-            g.ch = out.lh[j];
-            int y0 = -1, y1 = -1;
-            for (int y = 0; y < g.ch; y++)
-                if (rowf[y]) {
-                    if (y0 < 0) y0 = y;
-                    y1 = y;
-                }
-            int x0, len;
-            cyclic_interval(colf, g.cw, x0, len);
-            if (y0 < 0 || len == 0) {
-                g.x0 = g.y0 = g.bw = g.bh = 0;
-            } else {
-                g.x0 = x0;
-                g.bw = len;
-                g.y0 = y0;
-                g.bh = y1 - y0 + 1;
-            }
-            g.pitch = round_pitch(g.bw);
-        }
+        camera_boxes(L, out.lw.data(), out.lh.data(), std::move(colf), std::move(rowf),
+                     out.geom.data() + (size_t)i * (L + 1));
     });
 
     // validity (alpha != 0) per level on the boxes, MultiBandBlend.cpp:945-959:
@@ -393,107 +491,30 @@ int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const si
 
     // work lists
     const bool u8 = kind == kPixU8;
-    const int dtw = u8 ? kFTW : kDownTW, dth = u8 ? kFTH : kDownTH;
     const int ttw = u8 ? 2 * kQTW : kTileW, tth = u8 ? 2 * kQTH : kTileH;
-    out.down.assign(L, std::vector<DownJob>());
-    for (int j = 0; j < L; j++) {
-        for (int i = 0; i < n; i++) {
-            const Geom& gd = out.geom[(size_t)i * (L + 1) + j + 1];
-            if (gd.empty()) continue;
-            int segs[2][2] = {{gd.x0, std::min(gd.x0 + gd.bw, gd.cw)}, {0, gd.x0 + gd.bw - gd.cw}};
-            for (int sgi = 0; sgi < 2; sgi++) {
-                int a = segs[sgi][0], b = segs[sgi][1];
-                for (int X = a; X < b; X += dtw)
-                    for (int Y = gd.y0; Y < gd.y0 + gd.bh; Y += dth) {
-                        DownJob d{};
-                        d.src = (fused0 && j == 0) ? i : i * (L + 1) + j;
-                        d.dst = i * (L + 1) + j + 1;
-                        d.X = X;
-                        d.Y = Y;
-                        d.nx = std::min(dtw, b - X);
-                        d.ny = std::min(dth, gd.y0 + gd.bh - Y);
-                        // outputs whose alpha is zero stay zero: skip empty tiles
-                        if (!tile_has_valid(i, j + 1, d.X, d.Y, d.nx, d.ny)) continue;
-                        out.down[j].push_back(d);
-                    }
-            }
-        }
-    }
-    out.blend.assign(L + 1, std::vector<TileJob>());
-    out.camlist.clear();
-    out.max_tile_cams = 0;
-    for (int j = 1; j <= L; j++) {
-        int tw = (out.lw[j] + ttw - 1) / ttw, th = (out.lh[j] + tth - 1) / tth;
-        std::vector<std::vector<int>> cams((size_t)tw * th);
-        for (int i = 0; i < n; i++) {
-            const Geom& g = out.geom[(size_t)i * (L + 1) + j];
-            const std::vector<uint16_t>& w = out.weight[(size_t)i * (L + 1) + j];
-            std::vector<uint8_t> hit((size_t)tw * th, 0);
-            for (int ly = 0; ly < g.bh; ly++)
-                for (int lx = 0; lx < g.bw; lx++)
-                    if (w[(size_t)ly * g.pitch + lx]) {
-                        int X = (g.x0 + lx) % g.cw, Y = g.y0 + ly;
-                        hit[(size_t)(Y / tth) * tw + X / ttw] = 1;
-                    }
-            for (size_t t = 0; t < hit.size(); t++)
-                if (hit[t]) cams[t].push_back(i * (L + 1) + j);
-        }
-        for (int ty = 0; ty < th; ty++)
-            for (int tx = 0; tx < tw; tx++) {
-                TileJob t{};
-                t.X = tx * ttw;
-                t.Y = ty * tth;
-                t.nx = std::min(ttw, out.lw[j] - t.X);
-                t.ny = std::min(tth, out.lh[j] - t.Y);
-                t.r = 0;
-                const std::vector<int>& c = cams[(size_t)ty * tw + tx];
-                t.off = (int)out.camlist.size();
-                t.cnt = (int)c.size();
-                t.flags = j == L ? kTileTop : 0;
-                out.camlist.insert(out.camlist.end(), c.begin(), c.end());
-                out.max_tile_cams = std::max(out.max_tile_cams, t.cnt);
-                out.blend[j].push_back(t);
-            }
-    }
-    {
-        int tw = (cols + ttw - 1) / ttw, th = (rows + tth - 1) / tth;
-        out.fin_owners.clear();
-        for (int ty = 0; ty < th; ty++)
-            for (int tx = 0; tx < tw; tx++) {
-                TileJob t{};
-                t.X = tx * ttw;
-                t.Y = ty * tth;
-                t.nx = std::min(ttw, cols - t.X);
-                t.ny = std::min(tth, rows - t.Y);
-                t.off = (int)out.fin_owners.size();
-                if (u8) {
-                    uint8_t seen[256] = {};
-                    for (int y = t.Y; y < t.Y + t.ny; y++)
-                        for (int x = t.X; x < t.X + t.nx; x++) {
-                            uint8_t o = out.label[(size_t)y * cols + x];
-                            if (o != kNoOwner && !seen[o]) {
-                                seen[o] = 1;
-                                out.fin_owners.push_back(o);
-                            }
-                        }
+    auto cam_hits = [&](int j, int i, std::vector<uint8_t>& hit) {
+        const int tw = (out.lw[j] + ttw - 1) / ttw;
+        const Geom& g = out.geom[(size_t)i * (L + 1) + j];
+        const std::vector<uint16_t>& w = out.weight[(size_t)i * (L + 1) + j];
+        for (int ly = 0; ly < g.bh; ly++)
+            for (int lx = 0; lx < g.bw; lx++)
+                if (w[(size_t)ly * g.pitch + lx]) {
+                    int X = (g.x0 + lx) % g.cw, Y = g.y0 + ly;
+                    hit[(size_t)(Y / tth) * tw + X / ttw] = 1;
                 }
-                t.cnt = (int)out.fin_owners.size() - t.off;
-                out.fin.push_back(t);
+    };
+    auto tile_owners = [&](const TileJob& t, std::vector<int>& owners) {
+        uint8_t seen[256] = {};
+        for (int y = t.Y; y < t.Y + t.ny; y++)
+            for (int x = t.X; x < t.X + t.nx; x++) {
+                uint8_t o = out.label[(size_t)y * cols + x];
+                if (o != kNoOwner && !seen[o]) {
+                    seen[o] = 1;
+                    owners.push_back(o);
+                }
             }
-    }
-    for (int i = 0; i < n; i++) {
-        const Geom& g = out.geom[(size_t)i * (L + 1)];
-        for (int ly = 0; ly < g.bh; ly += kRemapTH)
-            for (int lx = 0; lx < g.bw; lx += kRemapTW) {
-                RemapJob r{};
-                r.cam = i;
-                r.lx = lx;
-                r.ly = ly;
-                r.nx = std::min(kRemapTW, g.bw - lx);
-                r.ny = std::min(kRemapTH, g.bh - ly);
-                out.remap.push_back(r);
-            }
-    }
+    };
+    work_lists(out, tile_has_valid, cam_hits, tile_owners);
     return PANO_OK;
 }
 

@@ -5,6 +5,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <functional>
 #include <thread>
 #include <vector>
 
@@ -67,6 +68,26 @@ void nonintersecting_masks(int n, int rows, int cols, const uint8_t* const* mask
                            const size_t* steps, uint8_t* out);
 
 int true_num_levels(int width, int height, int max_levels, int min_length);
+
+// Boxes of one camera per level 0..L: colf / rowf flag the level-0 columns /
+// rows its support meets; each level dilates them by the 5-tap footprint
+// (WRAP columns, REFLECT_101 rows) and takes the smallest cyclic column
+// interval and the row span.  lw / lh: level sizes.
+void camera_boxes(int L, const int* lw, const int* lh, std::vector<uint8_t> colf, std::vector<uint8_t> rowf,
+                  Geom* geom);
+
+// Work lists of a prepared rig from its geometry (n, rows, cols, levels, lw,
+// lh, geom, kind, fused0 set):
+//   tile_has_valid(i, j, X, Y, nx, ny): camera i's level-j alpha is non-zero
+//     somewhere in the canvas tile (down jobs of other tiles are dropped);
+//   cam_hits(j, i, hit): sets hit[tile] for the level-j blend tiles where camera
+//     i's weight is non-zero (row-major tile grid of the level canvas);
+//   tile_owners(t, owners): appends the owners of level-0 tile t (8-bit path).
+using TileValidFn = std::function<bool(int, int, int, int, int, int)>;
+using CamHitsFn = std::function<void(int, int, std::vector<uint8_t>&)>;
+using TileOwnersFn = std::function<void(const TileJob&, std::vector<int>&)>;
+void work_lists(Prepared& out, const TileValidFn& tile_has_valid, const CamHitsFn& cam_hits,
+                const TileOwnersFn& tile_owners);
 
 // Full prepare.  support0 (optional, n planes rows x cols): pixels whose level-0
 // value may be non-zero besides the mask (e.g. where a map is valid); with

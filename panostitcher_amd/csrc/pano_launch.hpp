@@ -1,7 +1,10 @@
 // pano_launch.hpp -- host-side launchers of the gfx950 kernels (kernels.hip).
 // Each returns hipSuccess or the launch error; no launch allocates memory.
 #pragma once
+#include <vector>
+
 #include "pano_dev.hpp"
+#include "pano_host.hpp"
 
 namespace pano {
 
@@ -108,10 +111,43 @@ hipError_t launch_reproject_accum(const uint8_t* src, size_t sstep, int sw, int 
 hipError_t launch_linear_blend32s(const SrcTable& images, int n, const int32_t* wts, int pitch, size_t cam_stride,
                                   uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
 
-// getNonIntersectingMasks on the device (seams.hip): host masks in, host
-// unique masks out (n planes rows x cols); synchronous.
+// getNonIntersectingMasks on device planes (pitch cols), masks replaced by the
+// unique masks; synchronous.
+hipError_t device_seams_inplace(int n, int rows, int cols, uint8_t* M);
+// The same with host masks in and host unique masks out (n planes rows x cols).
 hipError_t device_nonintersecting_masks(int n, int rows, int cols, const uint8_t* const* masks,
                                         const size_t* steps, uint8_t* out);
+
+// ---- render prepare on the device (prepare_dev.hip) ----------------------
+// Device results of prepare_tables_device, read by the rig build: weights of
+// every (camera, level >= 1) plane, box-local with the plane pitch, at element
+// woff[i (L + 1) + j] of w; per level the weight sums (pitch lw[j], only when
+// the canvas is not fully covered); owner per canvas pixel; mask bits per
+// canvas pixel (n <= kGainMaxCams, when asked for).
+struct DevTables {
+    uint16_t* w = nullptr;
+    std::vector<size_t> woff;
+    std::vector<uint16_t*> wsum;
+    uint8_t* label = nullptr;
+    uint8_t* cover = nullptr;
+    DevTables() = default;
+    DevTables(const DevTables&) = delete;
+    DevTables& operator=(const DevTables&) = delete;
+    ~DevTables();
+};
+// The prepare of the fused 8-bit render (boxes, work lists, weight pyramids,
+// weight sums, owners) from device masks and unique masks (n planes, pitch
+// cols), n <= 32; synchronous.  Fills out as prepare_rig(..., kPixU8, fused0)
+// does, except out.unique / label / weight / wsum, which stay on the device.
+int prepare_tables_device(int n, int rows, int cols, const uint8_t* masks, const uint8_t* unique, int max_levels,
+                          bool with_cover, Prepared& out, DevTables& dev);
+// Margined fused map storage of camera cam (box g at level 0) from device maps
+// and masks (pitch cols) and the owner plane.
+hipError_t launch_pack_map(const float* xm, const float* ym, const uint8_t* mask, const uint8_t* label, int cols,
+                           int cam, const Geom& g, int margin, int mp, int sw, int sh, float2* out, hipStream_t s);
+// Whether every valid map point (inside the source) lies inside the mask.
+int maps_consistent_device(const float* xm, const float* ym, const uint8_t* mask, size_t P, int sw, int sh,
+                           bool& consistent);
 
 hipError_t launch_copy_plane(const void* src, size_t src_step, void* dst, size_t dst_step,
                              int row_bytes, int rows, hipStream_t s);

@@ -154,25 +154,16 @@ struct Buf {
 
 }  // namespace
 
-// Host masks in, host unique masks out (n planes of rows x cols); synchronous.
-// Returns hipSuccess or the first error.
-hipError_t device_nonintersecting_masks(int n, int rows, int cols, const uint8_t* const* masks,
-                                        const size_t* steps, uint8_t* out)
+// Device masks in, device unique masks out, in place: n planes of rows x cols
+// bytes (pitch cols) at M; synchronous.
+hipError_t device_seams_inplace(int n, int rows, int cols, uint8_t* M)
 {
-    // planes packed (pitch = cols) so that host copies are one contiguous block
     const int pitch = cols;
     const size_t plane = (size_t)rows * pitch;
-    Buf dm, acc, g, du, di;
-    if (!dm.alloc(plane * n) || !acc.alloc(plane) || !g.alloc(plane * 4) || !du.alloc(plane * 4) ||
-        !di.alloc(plane * 4))
+    Buf acc, g, du, di;
+    if (!acc.alloc(plane) || !g.alloc(plane * 4) || !du.alloc(plane * 4) || !di.alloc(plane * 4))
         return hipErrorOutOfMemory;
-    uint8_t* M = (uint8_t*)dm.p;
-    hipError_t e = hipSuccess;
-    for (int i = 0; i < n && e == hipSuccess; i++)
-        e = steps[i] == (size_t)cols
-                ? hipMemcpy(M + plane * i, masks[i], plane, hipMemcpyHostToDevice)
-                : hipMemcpy2D(M + plane * i, pitch, masks[i], steps[i], cols, rows, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(acc.p, M, plane, hipMemcpyDeviceToDevice);
+    hipError_t e = hipMemcpy(acc.p, M, plane, hipMemcpyDeviceToDevice);
     const dim3 grid((cols + 63) / 64, (rows + 3) / 4);
     for (int i = 1; i < n && e == hipSuccess; i++) {
         e = dt((const uint8_t*)acc.p, pitch, cols, rows, (int32_t*)g.p, (int32_t*)du.p, nullptr);
@@ -183,6 +174,27 @@ hipError_t device_nonintersecting_masks(int n, int rows, int cols, const uint8_t
             e = hipGetLastError();
         }
     }
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    return e;
+}
+
+// Host masks in, host unique masks out (n planes of rows x cols); synchronous.
+// Returns hipSuccess or the first error.
+hipError_t device_nonintersecting_masks(int n, int rows, int cols, const uint8_t* const* masks,
+                                        const size_t* steps, uint8_t* out)
+{
+    // planes packed (pitch = cols) so that host copies are one contiguous block
+    const int pitch = cols;
+    const size_t plane = (size_t)rows * pitch;
+    Buf dm;
+    if (!dm.alloc(plane * n)) return hipErrorOutOfMemory;
+    uint8_t* M = (uint8_t*)dm.p;
+    hipError_t e = hipSuccess;
+    for (int i = 0; i < n && e == hipSuccess; i++)
+        e = steps[i] == (size_t)cols
+                ? hipMemcpy(M + plane * i, masks[i], plane, hipMemcpyHostToDevice)
+                : hipMemcpy2D(M + plane * i, pitch, masks[i], steps[i], cols, rows, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = device_seams_inplace(n, rows, cols, M);
     if (e == hipSuccess) e = hipMemcpy(out, M, plane * n, hipMemcpyDeviceToHost);
     return e;
 }

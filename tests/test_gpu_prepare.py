@@ -1,0 +1,91 @@
+"""The fused render's prepare built on the device (boxes, weight pyramids,
+weight sums, owners, cover bits, work lists, margined map storage) against
+the same prepare built on the host: the rig must be the same, so the renders
+are bit-identical and the work lists give the same per-stage byte counts and
+the same device footprint.  The host prepare's renders are bit-exact against
+the oracle in the parity tests; the device prepare is the default path there
+as well."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import panostitcher_amd as P  # noqa: E402
+from panostitcher_amd import synth  # noqa: E402
+from tests.gpu_util import from_dev, note, to_dev  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    P.lib()
+    yield
+
+
+def _render(r, wl, srcs, luts, autogain):
+    out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    if autogain:
+        assert r.render_autogain(srcs, out, tint=True)
+    else:
+        assert r.render(srcs, out, luts)
+    return from_dev(out)
+
+
+# mixed8: full coverage; rect6: partial coverage (weight sums); fisheye4 with
+# per-frame gains (cover bits); fisheye2: two cameras, small canvas
+@pytest.mark.parametrize("name,scale,autogain", [("mixed8", 0.25, False), ("rect6", 0.3, False),
+                                                 ("fisheye4", 0.15, True), ("fisheye2", 0.25, False)])
+def test_device_prepare_matches_host(name, scale, autogain):
+    wl = synth.workload(name, scale)
+    xs, ys, ms = synth.maps_and_masks(wl)
+    srcs = [to_dev(synth.bgrx(f)) for f in synth.frames(wl)]
+    luts = synth.luts(wl)
+    sizes = [(c.width, c.height) for c in wl.cameras]
+    res = []
+    for on in (False, True):
+        note(f"prepare {name}@{scale} on_device={on} {wl.dst_w}x{wl.dst_h} n={wl.n}")
+        r = P.PanoramaRender()
+        r.set_prepare_on_device(on)
+        assert r.prepare_maps(sizes, (wl.dst_w, wl.dst_h), xs, ys, ms, wl.levels)
+        assert r.fused
+        img = _render(r, wl, srcs, luts, autogain)
+        res.append((img, [r.model_bytes(k) for k in range(5)], r.mem_bytes(), r.prepare_ms()))
+    note(f"prepare ms host {res[0][3]} device {res[1][3]}")
+    np.testing.assert_array_equal(res[1][0], res[0][0])
+    assert res[1][1] == res[0][1]
+    assert res[1][2] == res[0][2]
+
+
+def test_device_prepare_from_params_matches_host():
+    wl = synth.workload("mixed8", 0.2)
+    srcs = [to_dev(synth.bgrx(f)) for f in synth.frames(wl)]
+    params = [c.param() for c in wl.cameras]
+    sizes = [(c.width, c.height) for c in wl.cameras]
+    imgs = []
+    for on in (False, True):
+        note(f"prepare from params on_device={on}")
+        r = P.PanoramaRender()
+        r.set_prepare_on_device(on)
+        assert r.prepare(params, sizes, (wl.dst_w, wl.dst_h), wl.levels)
+        imgs.append(_render(r, wl, srcs, None, False))
+    np.testing.assert_array_equal(imgs[1], imgs[0])
+
+
+def test_device_prepare_repeated_and_batched():
+    # a render prepared twice on the device, and two renders in one batch
+    wl = synth.workload("rect6", 0.25)
+    xs, ys, ms = synth.maps_and_masks(wl)
+    srcs = [to_dev(synth.bgrx(f)) for f in synth.frames(wl)]
+    sizes = [(c.width, c.height) for c in wl.cameras]
+    r = P.PanoramaRender()
+    r.set_prepare_on_device(False)
+    assert r.prepare_maps(sizes, (wl.dst_w, wl.dst_h), xs, ys, ms, wl.levels)
+    ref = _render(r, wl, srcs, None, False)
+    d = P.PanoramaRender()
+    for k in range(2):
+        note(f"device prepare, round {k}")
+        assert d.prepare_maps(sizes, (wl.dst_w, wl.dst_h), xs, ys, ms, wl.levels)
+        np.testing.assert_array_equal(_render(d, wl, srcs, None, False), ref)
