@@ -373,7 +373,22 @@ def run_batched(args):
                           "p99": round(float(np.percentile(lat, 99)), 4),
                           "streams": S,
                           "note": "device time of a step; each stream's frame completes with it"}
-    line["prepare"] = {"s_per_rig": round(prep_each, 3), "last_ms_by_phase": renders[-1].prepare_ms()}
+    line["prepare"] = {"s_per_rig": round(prep_each, 3), "last_ms_by_phase": renders[-1].prepare_ms(),
+                       "note": "host maps uploaded; tables, pyramids and map storage built on the device"}
+    # a rig prepared from its camera parameters (maps generated on the device),
+    # with the tables on the device and, for comparison, on the host
+    params = [c.param() for c in wl.cameras]
+    sizes = [(c.width, c.height) for c in wl.cameras]
+    for on, key in ((True, "from_params_s"), (False, "from_params_host_tables_s")):
+        pr = P.PanoramaRender()
+        pr.set_prepare_on_device(on)
+        t1 = time.perf_counter()
+        if not pr.prepare(params, sizes, (wl.dst_w, wl.dst_h), wl.levels):
+            raise SystemExit("prepare from params failed")
+        line["prepare"][key] = round(time.perf_counter() - t1, 3)
+        line["prepare"][key.replace("_s", "_ms_by_phase")] = pr.prepare_ms()
+        del pr
+    log(f"prepare {line['prepare']}")
     line["single_stream"] = single
     line["batched_eager"] = eager
     line["batched_graph"] = graph
@@ -457,6 +472,7 @@ def run_linear(args):
     assert lr.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h), xs, ys, ms, radius)
     prep_s = time.perf_counter() - t0
     log(f"linear prepare (radius {radius}) in {prep_s:.2f}s")
+    wts = [lr.weights(i) for i in range(wl.n)]   # read once, before any frame
     dsrc = [torch.from_numpy(s).cuda() for s in srcs]
     out = torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
     stream = torch.cuda.current_stream()
@@ -473,7 +489,7 @@ def run_linear(args):
     # the source pixel's 3 bytes per sampled one, the 4-byte output per canvas pixel
     listed = sampled = 0
     for i in range(wl.n):
-        w = lr.weights(i)
+        w = wts[i]
         sampled += int(((w > 0) & (xs[i] >= 0) & (ys[i] >= 0) & (xs[i] < wl.cameras[i].width)
                         & (ys[i] < wl.cameras[i].height)).sum())
         m = ms[i] != 0
@@ -504,7 +520,7 @@ def run_linear(args):
         from oracle import oracle as O
 
         threads = min(16, os.cpu_count() or 1)
-        wts = np.stack([lr.weights(i) for i in range(wl.n)])
+        wts = np.stack(wts)
         bgr = [s[..., :3].copy() for s in srcs]
         t0 = time.perf_counter()
         for _ in range(args.cpu_frames):
