@@ -231,16 +231,11 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
     for (int round = 0; round < kRounds; round++) {
         const int cur = round & 1;
         uint32_t px[kBatch][4];
-        uint32_t o[kBatch][4];
 #pragma unroll
         for (int b = 0; b < kBatch; b++) {
             const float x = __builtin_fabsf(m[cur][b].x), y = __builtin_fabsf(m[cur][b].y);
             px[b][0] = px[b][1] = px[b][2] = px[b][3] = 0;
-            if (x < fsw && y < fsh) {
-                tap_offsets_fast(sstep, cam.sw, cam.sh, x, y, o[b]);
-#pragma unroll
-                for (int q = 0; q < 4; q++) px[b][q] = ldg((const uint32_t*)(sp + o[b][q]));
-            }
+            if (x < fsw && y < fsh) load_taps(sp, sstep, cam.sw, cam.sh, x, y, px[b]);
         }
         if (round + 1 < kRounds) fetch_map(round + 1, m[cur ^ 1]);
 #pragma unroll

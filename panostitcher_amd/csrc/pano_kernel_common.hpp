@@ -276,6 +276,32 @@ __device__ __forceinline__ void tap_offsets_fast(uint32_t sstep, int sw, int sh,
     o[3] = o0 + sstep + 4u;
 }
 
+// The four bilinear taps of (x, y), a point inside the image: away from the
+// last row and column two 8-byte loads (a pixel and its right neighbour, on a
+// 4-byte boundary: the hardware takes dword-aligned multi-dword loads), else
+// the four loads at tap_offsets' edge rules.
+typedef uint32_t pano_u32x2 __attribute__((ext_vector_type(2)));
+typedef pano_u32x2 pano_u32x2_a4 __attribute__((aligned(4)));
+__device__ __forceinline__ void load_taps(const uint8_t* sp, uint32_t sstep, int sw, int sh, float x, float y,
+                                          uint32_t px[4])
+{
+    const int x0 = (int)x, y0 = (int)y;
+    if (x0 + 1 < sw && y0 + 1 < sh) {
+        const uint32_t o0 = __umul24((uint32_t)y0, sstep) + 4u * (uint32_t)x0;
+        const pano_u32x2 a = *(const PANO_GLOBAL pano_u32x2_a4*)(sp + o0);
+        const pano_u32x2 c = *(const PANO_GLOBAL pano_u32x2_a4*)(sp + o0 + sstep);
+        px[0] = a.x;
+        px[1] = a.y;
+        px[2] = c.x;
+        px[3] = c.y;
+        return;
+    }
+    uint32_t o[4];
+    tap_offsets(sstep, sw, sh, x, y, o);
+#pragma unroll
+    for (int q = 0; q < 4; q++) px[q] = ldg((const uint32_t*)(sp + o[q]));
+}
+
 // BT.601 studio swing in 20-bit fixed point: OpenCV's cvtColor table, which the
 // encoder hand-off uses (CudaAccel/ImageProc.cu:44-60).  Forward: Y of every
 // pixel of a 2x2 block, U and V from its left column (top and bottom averaged

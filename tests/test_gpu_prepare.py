@@ -89,3 +89,31 @@ def test_device_prepare_repeated_and_batched():
         note(f"device prepare, round {k}")
         assert d.prepare_maps(sizes, (wl.dst_w, wl.dst_h), xs, ys, ms, wl.levels)
         np.testing.assert_array_equal(_render(d, wl, srcs, None, False), ref)
+
+
+@pytest.mark.parametrize("case", ["empty_camera", "single_camera"])
+def test_device_prepare_edge_rigs(case):
+    # a camera whose mask is empty (its maps invalid everywhere), and a rig of
+    # one camera (partial coverage)
+    wl = synth.workload("mixed8" if case == "empty_camera" else "fisheye2", 0.15)
+    xs, ys, ms = (list(v) for v in synth.maps_and_masks(wl))
+    frames = synth.frames(wl)
+    sizes = [(c.width, c.height) for c in wl.cameras]
+    if case == "empty_camera":
+        xs[3] = np.full_like(xs[3], -1)
+        ys[3] = np.full_like(ys[3], -1)
+        ms[3] = np.zeros_like(ms[3])
+    else:
+        xs, ys, ms, frames, sizes = xs[:1], ys[:1], ms[:1], frames[:1], sizes[:1]
+    srcs = [to_dev(synth.bgrx(f)) for f in frames]
+    imgs = []
+    for on in (False, True):
+        note(f"edge rig {case} on_device={on}")
+        r = P.PanoramaRender()
+        r.set_prepare_on_device(on)
+        assert r.prepare_maps(sizes, (wl.dst_w, wl.dst_h), xs, ys, ms, wl.levels)
+        out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+        assert r.render(srcs, out)
+        imgs.append((from_dev(out), [r.model_bytes(k) for k in range(5)]))
+    np.testing.assert_array_equal(imgs[1][0], imgs[0][0])
+    assert imgs[1][1] == imgs[0][1]
