@@ -508,6 +508,12 @@ def run_linear(args):
                                   f"{wl.n} cams, one frame per step", "streams": 1,
                       "parallelism": "single GPU"})
     line["prepare_s"] = round(prep_s, 3)
+    lp = P.LinearRender()
+    t0 = time.perf_counter()
+    assert lp.prepare([c.param() for c in wl.cameras], [(c.width, c.height) for c in wl.cameras],
+                      (wl.dst_w, wl.dst_h), radius)
+    line["prepare_from_params_s"] = round(time.perf_counter() - t0, 3)
+    del lp
     line["roofline"] = {"bound": "hbm", "kernel": "linear_render", "achieved": round(achieved, 1),
                         "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                         "traffic": (read_traffic(f"{wl.name}_linear") or {}).get("linear_render"),

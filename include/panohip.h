@@ -215,6 +215,11 @@ void pano_linear_render_destroy(pano_linear_render* r);
 int pano_linear_render_prepare_maps(pano_linear_render* r, int num, const int* src_w, const int* src_h, int dst_w,
                                     int dst_h, const float* const* xmaps, const float* const* ymaps,
                                     const uint8_t* const* masks, int radius);
+/* The same from camera parameters: maps and masks generated on the device
+ * (pano_generate_map_device), then prepared there (CudaPanoramaRender2::prepare
+ * with highQualityBlend = 0, RicohUtil.cpp:1175-1180). */
+int pano_linear_render_prepare(pano_linear_render* r, int num, const pano_photo_param* params, const int* src_w,
+                               const int* src_h, int dst_w, int dst_h, int radius);
 int pano_linear_render_render(pano_linear_render* r, int num, const uint8_t* const* srcs, const size_t* src_steps,
                               const uint8_t* luts, uint8_t* dst, size_t dst_step, hipStream_t stream);
 /* The prepared float weights of camera cam (host dst_w x dst_h floats). */

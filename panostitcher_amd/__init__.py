@@ -94,6 +94,7 @@ def lib():
             "pano_linear_render_create": ([pp], i),
             "pano_linear_render_destroy": ([vp], None),
             "pano_linear_render_prepare_maps": ([vp, i, vp, vp, i, i, pp, pp, pp, i], i),
+            "pano_linear_render_prepare": ([vp, i, vp, vp, vp, i, i, i], i),
             "pano_linear_render_render": ([vp, i, pp, psz, vp, vp, sz, vp], i),
             "pano_linear_render_weights": ([vp, i, vp], i),
             "pano_blender32f_create": ([pp], i),
@@ -196,7 +197,7 @@ def exported_symbols():
         "pano_reproject_weighted_accumulate", "pano_reproject_params",
         "pano_load_photo_params", "pano_load_photo_params_xml", "pano_load_photo_params_pts",
         "pano_export_photo_params_xml", "pano_rotate_cameras", "pano_rotate_photo_params_xml",
-        "pano_render_set_prepare_on_device",
+        "pano_render_set_prepare_on_device", "pano_linear_render_prepare",
     ]
 
 
@@ -579,6 +580,26 @@ class LinearRender:
                                                    _ptrs([a.ctypes.data for a in xs]),
                                                    _ptrs([a.ctypes.data for a in ys]),
                                                    _ptrs([a.ctypes.data for a in ms]), int(radius))
+        if st == PANO_OK:
+            self.n, self.dst = n, (dw, dh)
+        return st == PANO_OK
+
+    def prepare(self, params, src_sizes, dst_size, radius: int) -> bool:
+        """From camera parameters (PhotoParams or a rig file path): maps and masks
+        generated and prepared on the device.  src_sizes: one (w, h) per camera,
+        or one for all."""
+        if isinstance(params, (str, bytes, os.PathLike)):
+            params = load_photo_params(params)
+            if not params:
+                return False
+        n = len(params)
+        if len(src_sizes) == 2 and all(isinstance(v, (int, np.integer)) for v in src_sizes):
+            src_sizes = [tuple(src_sizes)] * n
+        dw, dh = dst_size
+        arr = (PhotoParam * n)(*params)
+        sw = (C.c_int * n)(*[s[0] for s in src_sizes])
+        sh = (C.c_int * n)(*[s[1] for s in src_sizes])
+        st = lib().pano_linear_render_prepare(self._h, n, C.cast(arr, C.c_void_p), sw, sh, dw, dh, int(radius))
         if st == PANO_OK:
             self.n, self.dst = n, (dw, dh)
         return st == PANO_OK

@@ -197,9 +197,45 @@ __global__ __launch_bounds__(256) void k_reproject_accum(const uint8_t* __restri
     p[3] = 0.f;
 }
 
+// x and y planes (pitch sp elements) into (x, y) pairs (pitch dp pairs).
+__global__ __launch_bounds__(256) void k_interleave_map(const float* __restrict__ xm, const float* __restrict__ ym,
+                                                        int sp, float2* __restrict__ out, int dp, int w, int h)
+{
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63), Y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (X >= w || Y >= h) return;
+    const size_t q = (size_t)Y * sp + X;
+    out[(size_t)Y * dp + X] = make_float2(ldg(xm + q), ldg(ym + q));
+}
+
+// hit[(tile) n + i] = 1 for the kLinTW x kLinTH canvas tiles where mask i
+// (n planes of pitch w, plane bytes apart) is set; hit zero on entry.
+__global__ __launch_bounds__(256) void k_lin_tile_hits(const uint8_t* __restrict__ masks, size_t plane, int n, int w,
+                                                       int h, int tw, uint8_t* __restrict__ hit)
+{
+    const int X = blockIdx.x * 64 + (threadIdx.x & 63), Y = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.z;
+    if (X >= w || Y >= h) return;
+    if (ldg(masks + plane * i + (size_t)Y * w + X)) hit[((size_t)(Y / kLinTH) * tw + X / kLinTW) * n + i] = 1;
+}
+
 namespace {
 dim3 grid_px(int w, int h) { return dim3((w + 63) / 64, (h + 3) / 4); }
 }  // namespace
+
+hipError_t launch_interleave_map(const float* xm, const float* ym, int sp, float2* out, int dp, int w, int h,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_interleave_map, grid_px(w, h), dim3(256), 0, s, xm, ym, sp, out, dp, w, h);
+    return hipGetLastError();
+}
+
+hipError_t launch_lin_tile_hits(const uint8_t* masks, size_t plane, int n, int w, int h, int tw, uint8_t* hit,
+                                hipStream_t s)
+{
+    dim3 g = grid_px(w, h);
+    g.z = n;
+    hipLaunchKernelGGL(k_lin_tile_hits, g, dim3(256), 0, s, masks, plane, n, w, h, tw, hit);
+    return hipGetLastError();
+}
 
 hipError_t launch_reproject_accum(const uint8_t* src, size_t sstep, int sw, int sh, const float* xmap,
                                   const float* ymap, size_t map_step, const float* wt, size_t wstep, float* acc,

@@ -25,6 +25,12 @@
 
 using namespace pano;
 
+namespace pano {
+// maps.cpp: getReprojectMap32FAndMask on the device (cudaGenerateReprojectMapAndMask).
+int generate_map_device(const pano_photo_param& p, int src_w, int src_h, int dst_w, int dst_h, float* xmap,
+                        float* ymap, size_t map_step, uint8_t* mask, size_t mask_step, hipStream_t s);
+}  // namespace pano
+
 #define L_TRY(expr)                                   \
     do {                                              \
         hipError_t e_ = (expr);                       \
@@ -91,34 +97,38 @@ int check_masks(int n, int rows, int cols, const uint8_t* const* masks, const si
     return PANO_OK;
 }
 
-// Weights of n host masks (rows x cols) into out: n device planes of rows x
-// pitch elements (float or int32), rows * pitch elements apart.  Synchronous.
-int device_weights(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps, int radius,
-                   bool as_int, int pitch, void* out)
+// Weights of n cameras from device masks (n planes of rows x cols bytes,
+// pitch cols): seams on the device, each unique mask blurred, ANDed with its
+// mask, normalised.  out: n planes of rows x pitch elements (float or int32),
+// rows * pitch elements apart.  Synchronous.
+int device_weights_dev(int n, int rows, int cols, const uint8_t* dmasks, int radius, bool as_int, int pitch,
+                       void* out)
 {
-    const size_t P = (size_t)rows * pitch;
-    std::vector<uint8_t> unique((size_t)rows * cols * n), host(P * n, 0), uhost(P * n, 0);
-    if (device_nonintersecting_masks(n, rows, cols, masks, steps, unique.data()) != hipSuccess) return PANO_ERR_HIP;
-    for (int i = 0; i < n; i++)
-        for (int y = 0; y < rows; y++) {
-            memcpy(&host[P * i + (size_t)y * pitch], masks[i] + (size_t)y * steps[i], cols);
-            memcpy(&uhost[P * i + (size_t)y * pitch], &unique[(size_t)rows * cols * i + (size_t)y * cols], cols);
-        }
+    const size_t P = (size_t)rows * pitch, Pc = (size_t)rows * cols;
     std::vector<int> K;
     const int nt = 2 * radius + 1;
     gaussian_taps(nt, radius / 3.0, K);
     int k0 = 0, k1 = nt;
     while (k0 < k1 && K[k0] == 0) k0++;
     while (k1 > k0 && K[k1 - 1] == 0) k1--;
-    DevBuf dmask, duniq, ddist, drows, dK;
-    int e = dmask.alloc(P * n);
+    DevBuf packed, dmask, duniq, ddist, drows, dK;
+    int e = packed.alloc(Pc * n);
+    if (!e) e = dmask.alloc(P * n);
     if (!e) e = duniq.alloc(P * n);
     if (!e) e = ddist.alloc(P * n);
     if (!e) e = drows.alloc(P * sizeof(int32_t));
     if (!e) e = dK.alloc(sizeof(int) * nt);
     if (e) return e;
-    L_TRY(hipMemcpy(dmask.p, host.data(), P * n, hipMemcpyHostToDevice));
-    L_TRY(hipMemcpy(duniq.p, uhost.data(), P * n, hipMemcpyHostToDevice));
+    L_TRY(hipMemcpy(packed.p, dmasks, Pc * n, hipMemcpyDeviceToDevice));
+    L_TRY(device_seams_inplace(n, rows, cols, packed.as<uint8_t>()));
+    L_TRY(hipMemset(dmask.p, 0, P * n));
+    L_TRY(hipMemset(duniq.p, 0, P * n));
+    for (int i = 0; i < n; i++) {
+        L_TRY(hipMemcpy2D(dmask.as<uint8_t>() + P * i, pitch, dmasks + Pc * i, cols, cols, rows,
+                          hipMemcpyDeviceToDevice));
+        L_TRY(hipMemcpy2D(duniq.as<uint8_t>() + P * i, pitch, packed.as<uint8_t>() + Pc * i, cols, cols, rows,
+                          hipMemcpyDeviceToDevice));
+    }
     L_TRY(hipMemcpy(dK.p, K.data(), sizeof(int) * nt, hipMemcpyHostToDevice));
     for (int i = 0; i < n; i++)
         L_TRY(launch_blur_u8(duniq.as<uint8_t>() + P * i, pitch, cols, rows, dK.as<int>(), nt, k0, k1,
@@ -127,6 +137,19 @@ int device_weights(int n, int rows, int cols, const uint8_t* const* masks, const
     L_TRY(launch_linear_weights(as_int, ddist.as<uint8_t>(), pitch, P, n, cols, rows, out, nullptr));
     L_TRY(hipDeviceSynchronize());
     return PANO_OK;
+}
+
+// The same from host masks (n planes with the given row steps).
+int device_weights(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps, int radius,
+                   bool as_int, int pitch, void* out)
+{
+    const size_t Pc = (size_t)rows * cols;
+    DevBuf dm;
+    int e = dm.alloc(Pc * n);
+    if (e) return e;
+    for (int i = 0; i < n; i++)
+        L_TRY(hipMemcpy2D(dm.as<uint8_t>() + Pc * i, cols, masks[i], steps[i], cols, rows, hipMemcpyHostToDevice));
+    return device_weights_dev(n, rows, cols, dm.as<uint8_t>(), radius, as_int, pitch, out);
 }
 
 int canvas_pitch(int cols) { return (cols + 63) / 64 * 64; }
@@ -239,44 +262,33 @@ int pano_linear_render_create(pano_linear_render** out)
 
 void pano_linear_render_destroy(pano_linear_render* r) { delete r; }
 
-int pano_linear_render_prepare_maps(pano_linear_render* r, int num, const int* src_w, const int* src_h, int dst_w,
-                                    int dst_h, const float* const* xmaps, const float* const* ymaps,
-                                    const uint8_t* const* masks, int radius)
+}  // extern "C"
+
+namespace {
+
+// The render's prepare from device maps (camera i: x plane at 2 i P, y plane
+// at (2 i + 1) P, pitch dst_w, P = dst_w dst_h) and device masks (n planes of
+// P bytes): weights, (x, y) map pairs, per-tile camera lists.  Arguments are
+// checked by the callers.
+int linear_prepare_dev(pano_linear_render* r, int num, const int* src_w, const int* src_h, int dst_w, int dst_h,
+                       const float* dmaps, const uint8_t* dmasks, int radius)
 {
-    if (!r) return PANO_ERR_ARG;
-    r->ok = false;
-    if (num <= 0 || num > kMaxSrc || !src_w || !src_h || !xmaps || !ymaps || !masks || radius < 0)
-        return PANO_ERR_ARG;
-    if (dst_w <= 0 || dst_h <= 0 || (size_t)dst_w * 4 >= (1u << 24)) return PANO_ERR_ARG;
-    for (int i = 0; i < num; i++)
-        if (src_w[i] <= 0 || src_h[i] <= 0 || !xmaps[i] || !ymaps[i] || !masks[i]) return PANO_ERR_ARG;
     const int pitch = canvas_pitch(dst_w);
-    const size_t P = (size_t)dst_h * pitch;
+    const size_t P = (size_t)dst_h * pitch, Pc = (size_t)dst_h * dst_w;
     r->n = num;
     r->dst_w = dst_w;
     r->dst_h = dst_h;
     r->pitch = pitch;
     r->src_w.assign(src_w, src_w + num);
     r->src_h.assign(src_h, src_h + num);
-    std::vector<size_t> msteps(num, (size_t)dst_w);
     int e = r->wts.alloc(P * 4 * num);
     if (!e) e = r->maps.alloc(P * 8 * num);
     if (e) return e;
-    e = device_weights(num, dst_h, dst_w, masks, msteps.data(), radius, false, pitch, r->wts.p);
+    e = device_weights_dev(num, dst_h, dst_w, dmasks, radius, false, pitch, r->wts.p);
     if (e) return e;
-    // maps as (x, y) pairs, full canvas per camera
-    std::vector<float> xy((size_t)dst_w * 2);
     for (int i = 0; i < num; i++)
-        for (int y = 0; y < dst_h; y++) {
-            const float* xs = xmaps[i] + (size_t)y * dst_w;
-            const float* ys = ymaps[i] + (size_t)y * dst_w;
-            for (int x = 0; x < dst_w; x++) {
-                xy[2 * x] = xs[x];
-                xy[2 * x + 1] = ys[x];
-            }
-            L_TRY(hipMemcpy(r->maps.as<float>() + 2 * (P * i + (size_t)y * pitch), xy.data(),
-                            sizeof(float) * 2 * dst_w, hipMemcpyHostToDevice));
-        }
+        L_TRY(launch_interleave_map(dmaps + 2 * Pc * i, dmaps + (2 * (size_t)i + 1) * Pc, dst_w,
+                                    (float2*)(r->maps.as<float>() + 2 * P * i), pitch, dst_w, dst_h, nullptr));
     std::vector<LinCam> cams(num);
     for (int i = 0; i < num; i++) {
         cams[i].map = (const float2*)(r->maps.as<float>() + 2 * P * i);
@@ -289,12 +301,14 @@ int pano_linear_render_prepare_maps(pano_linear_render* r, int num, const int* s
     // tiles with the cameras whose mask (which bounds the weight) meets them, in camera order
     const int tw = (dst_w + kLinTW - 1) / kLinTW, th = (dst_h + kLinTH - 1) / kLinTH;
     std::vector<uint8_t> hit((size_t)tw * th * num, 0);
-    for (int i = 0; i < num; i++)
-        for (int y = 0; y < dst_h; y++) {
-            const uint8_t* m = masks[i] + (size_t)y * dst_w;
-            for (int x = 0; x < dst_w; x++)
-                if (m[x]) hit[((size_t)(y / kLinTH) * tw + x / kLinTW) * num + i] = 1;
-        }
+    {
+        DevBuf dhit;
+        e = dhit.alloc(hit.size());
+        if (e) return e;
+        L_TRY(hipMemset(dhit.p, 0, hit.size()));
+        L_TRY(launch_lin_tile_hits(dmasks, Pc, num, dst_w, dst_h, tw, dhit.as<uint8_t>(), nullptr));
+        L_TRY(hipMemcpy(hit.data(), dhit.p, hit.size(), hipMemcpyDeviceToHost));
+    }
     std::vector<LinTile> tiles;
     std::vector<int> camlist;
     for (int ty = 0; ty < th; ty++)
@@ -325,8 +339,67 @@ int pano_linear_render_prepare_maps(pano_linear_render* r, int num, const int* s
     r->lsrc.clear();
     r->lsrc.resize(num);
     r->lstep.assign(num, 0);
+    L_TRY(hipDeviceSynchronize());
     r->ok = true;
     return PANO_OK;
+}
+
+int linear_prepare_args(int num, const int* src_w, const int* src_h, int dst_w, int dst_h, int radius)
+{
+    if (num <= 0 || num > kMaxSrc || !src_w || !src_h || radius < 0) return PANO_ERR_ARG;
+    if (dst_w <= 0 || dst_h <= 0 || (size_t)dst_w * 4 >= (1u << 24)) return PANO_ERR_ARG;
+    for (int i = 0; i < num; i++)
+        if (src_w[i] <= 0 || src_h[i] <= 0) return PANO_ERR_ARG;
+    return PANO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pano_linear_render_prepare_maps(pano_linear_render* r, int num, const int* src_w, const int* src_h, int dst_w,
+                                    int dst_h, const float* const* xmaps, const float* const* ymaps,
+                                    const uint8_t* const* masks, int radius)
+{
+    if (!r) return PANO_ERR_ARG;
+    r->ok = false;
+    int e = linear_prepare_args(num, src_w, src_h, dst_w, dst_h, radius);
+    if (e) return e;
+    if (!xmaps || !ymaps || !masks) return PANO_ERR_ARG;
+    for (int i = 0; i < num; i++)
+        if (!xmaps[i] || !ymaps[i] || !masks[i]) return PANO_ERR_ARG;
+    const size_t Pc = (size_t)dst_h * dst_w;
+    DevBuf dmaps, dmasks;
+    e = dmaps.alloc(Pc * 8 * num);
+    if (!e) e = dmasks.alloc(Pc * num);
+    if (e) return e;
+    for (int i = 0; i < num; i++) {
+        L_TRY(hipMemcpy(dmaps.as<float>() + 2 * Pc * i, xmaps[i], Pc * 4, hipMemcpyHostToDevice));
+        L_TRY(hipMemcpy(dmaps.as<float>() + (2 * (size_t)i + 1) * Pc, ymaps[i], Pc * 4, hipMemcpyHostToDevice));
+        L_TRY(hipMemcpy(dmasks.as<uint8_t>() + Pc * i, masks[i], Pc, hipMemcpyHostToDevice));
+    }
+    return linear_prepare_dev(r, num, src_w, src_h, dst_w, dst_h, dmaps.as<float>(), dmasks.as<uint8_t>(), radius);
+}
+
+int pano_linear_render_prepare(pano_linear_render* r, int num, const pano_photo_param* params, const int* src_w,
+                               const int* src_h, int dst_w, int dst_h, int radius)
+{
+    if (!r) return PANO_ERR_ARG;
+    r->ok = false;
+    int e = linear_prepare_args(num, src_w, src_h, dst_w, dst_h, radius);
+    if (e) return e;
+    if (!params) return PANO_ERR_ARG;
+    const size_t Pc = (size_t)dst_h * dst_w;
+    DevBuf dmaps, dmasks;
+    e = dmaps.alloc(Pc * 8 * num);
+    if (!e) e = dmasks.alloc(Pc * num);
+    if (e) return e;
+    for (int i = 0; i < num && !e; i++)
+        e = generate_map_device(params[i], src_w[i], src_h[i], dst_w, dst_h, dmaps.as<float>() + 2 * Pc * i,
+                                dmaps.as<float>() + (2 * (size_t)i + 1) * Pc, (size_t)dst_w * 4,
+                                dmasks.as<uint8_t>() + Pc * i, (size_t)dst_w, nullptr);
+    if (e) return e;
+    return linear_prepare_dev(r, num, src_w, src_h, dst_w, dst_h, dmaps.as<float>(), dmasks.as<uint8_t>(), radius);
 }
 
 int pano_linear_render_render(pano_linear_render* r, int num, const uint8_t* const* srcs, const size_t* src_steps,

@@ -103,6 +103,10 @@ hipError_t launch_blur_u8(const uint8_t* src, int pitch, int w, int h, const int
                           int32_t* rows, const uint8_t* mask, int mpitch, uint8_t* dst, hipStream_t s);
 hipError_t launch_linear_weights(bool as_int, const uint8_t* dist, int pitch, size_t cam_stride, int n, int w,
                                  int h, void* out, hipStream_t s);
+hipError_t launch_interleave_map(const float* xm, const float* ym, int sp, float2* out, int dp, int w, int h,
+                                 hipStream_t s);
+hipError_t launch_lin_tile_hits(const uint8_t* masks, size_t plane, int n, int w, int h, int tw, uint8_t* hit,
+                                hipStream_t s);
 hipError_t launch_linear_render(const LinTile* tiles, int ntiles, const int* camlist, const LinCam* cams,
                                 const SrcTable& src, uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
 hipError_t launch_reproject_accum(const uint8_t* src, size_t sstep, int sw, int sh, const float* xmap,

@@ -85,3 +85,27 @@ def test_linear_render_matches_oracle(name, scale, r):
     ref0 = O.linear_render(srcs, xs, ys, w, None)
     assert lr.render(dsrc, out, None)
     assert_parity(from_dev(out)[..., :3], ref0, exact=True, label=f"linear {name} no luts")
+
+
+def test_linear_render_from_params_matches_oracle():
+    # maps generated on the device from camera parameters; the oracle runs on
+    # those same maps and masks (read back)
+    wl = synth.workload("rect6", 0.2)
+    srcs = synth.frames(wl)
+    r = 6
+    params = [c.param() for c in wl.cameras]
+    sizes = [(c.width, c.height) for c in wl.cameras]
+    note(f"linear render from params rect6@0.2 {wl.dst_w}x{wl.dst_h}")
+    lr = P.LinearRender()
+    assert lr.prepare(params, sizes, (wl.dst_w, wl.dst_h), r)
+    maps = [P.generate_map_device(q, w, h, wl.dst_w, wl.dst_h) for q, (w, h) in zip(params, sizes)]
+    xs = [from_dev(m[0]) for m in maps]
+    ys = [from_dev(m[1]) for m in maps]
+    ms = [from_dev(m[2]) for m in maps]
+    w = O.linear_weights(np.stack(ms), r)
+    for i in range(wl.n):
+        np.testing.assert_array_equal(lr.weights(i), w[i])
+    ref = O.linear_render(srcs, xs, ys, w, None)
+    out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    assert lr.render([to_dev(synth.bgrx(s)) for s in srcs], out)
+    assert_parity(from_dev(out)[..., :3], ref, exact=True, label="linear from params")
