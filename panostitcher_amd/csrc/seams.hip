@@ -66,6 +66,54 @@ __global__ __launch_bounds__(kDtT) void k_dt_rows(const uint8_t* __restrict__ m,
     }
 }
 
+// The same row pass with the row staged through LDS (rows up to kDtRowMax
+// pixels): the mask row is read and the distances written with consecutive
+// threads on consecutive pixels; the per-thread chunks run in LDS.
+constexpr int kDtRowMax = 8192;
+__global__ __launch_bounds__(kDtT) void k_dt_rows_lds(const uint8_t* __restrict__ m, int pitch, int w,
+                                                      int32_t* __restrict__ g)
+{
+    __shared__ uint8_t row[kDtRowMax];
+    __shared__ int32_t dist[kDtRowMax];
+    __shared__ int lastz[kDtT], nextz[kDtT];
+    const int y = blockIdx.x, t = threadIdx.x;
+    const uint8_t* src = m + (size_t)y * pitch;
+    for (int x = t; x < w; x += kDtT) row[x] = ldg(src + x);
+    __syncthreads();
+    const int chunk = (w + kDtT - 1) / kDtT;
+    const int x0 = min(t * chunk, w), x1 = min(x0 + chunk, w);
+    int lz = -1, nz = 1 << 30;
+    for (int x = x0; x < x1; x++)
+        if (!row[x]) {
+            lz = x;
+            if (nz == 1 << 30) nz = x;
+        }
+    lastz[t] = lz;
+    nextz[t] = nz;
+    __syncthreads();
+    for (int s = 1; s < kDtT; s <<= 1) {
+        const int a = t >= s ? lastz[t - s] : -1;
+        const int b = t + s < kDtT ? nextz[t + s] : 1 << 30;
+        __syncthreads();
+        lastz[t] = max(lastz[t], a);
+        nextz[t] = min(nextz[t], b);
+        __syncthreads();
+    }
+    int prev = t > 0 ? lastz[t - 1] : -1;
+    int nx = t + 1 < kDtT ? nextz[t + 1] : 1 << 30;
+    for (int x = x0; x < x1; x++) {
+        if (!row[x]) prev = x;
+        dist[x] = prev >= 0 ? x - prev : kInf;
+    }
+    for (int x = x1 - 1; x >= x0; x--) {
+        if (!row[x]) nx = x;
+        if (nx < (1 << 30)) dist[x] = min(dist[x], nx - x);
+    }
+    __syncthreads();
+    int32_t* out = g + (size_t)y * pitch;
+    for (int x = t; x < w; x += kDtT) out[x] = dist[x];
+}
+
 // Column pass and cap: d = min(min_y' g(y') + |y - y'|, 8192 + distance to the
 // frame).  A block holds 64 columns and kColSeg row segments; each thread sweeps
 // its segment down and up, and the segments' boundary values (the minimum of
@@ -138,7 +186,10 @@ __global__ __launch_bounds__(256) void k_seam_step(const int32_t* __restrict__ d
 
 hipError_t dt(const uint8_t* m, int pitch, int w, int h, int32_t* g, int32_t* d, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dt_rows, dim3(h), dim3(kDtT), 0, s, m, pitch, w, g);
+    if (w <= kDtRowMax)
+        hipLaunchKernelGGL(k_dt_rows_lds, dim3(h), dim3(kDtT), 0, s, m, pitch, w, g);
+    else
+        hipLaunchKernelGGL(k_dt_rows, dim3(h), dim3(kDtT), 0, s, m, pitch, w, g);
     hipLaunchKernelGGL(k_dt_cols, dim3((w + 63) / 64), dim3(64 * kColSeg), 0, s, g, pitch, w, h, d);
     return hipGetLastError();
 }

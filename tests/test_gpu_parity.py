@@ -60,8 +60,9 @@ def test_reproject_to16s_matches_oracle():
     assert not got8[..., 3].any()
 
 
+# rows up to 8192 pixels take the LDS-staged row pass, wider ones the chunked one
 @pytest.mark.parametrize("seed,n,h,w", [(0, 3, 40, 96), (1, 4, 17, 2000), (2, 1, 9, 13), (3, 5, 64, 300),
-                                        (4, 3, 1, 77)])
+                                        (4, 3, 1, 77), (5, 3, 6, 9000), (6, 2, 5, 8192)])
 def test_nonintersecting_masks_device_matches_oracle(seed, n, h, w):
     rng = np.random.default_rng(seed)
     masks = random_masks(rng, n, h, w) if h > 3 else (rng.random((n, h, w)) < 0.7).astype(np.uint8) * 255
