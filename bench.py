@@ -87,17 +87,21 @@ def stage_model_bytes(wl, xs, ys, ms, L):
     return st
 
 
-def cpu_baseline(wl, srcs, xs, ys, ms, luts, frames: int):
+def cpu_baseline(wl, srcs, xs, ys, ms, luts, frames: int, seconds: float = 10.0):
     from oracle import oracle as O
 
     threads = min(16, os.cpu_count() or 1)
     ob = O.Blender()
     ob.prepare(np.stack(ms), wl.levels)
     bgr = [s[..., :3].copy() for s in srcs]
+    # whole frames until about `seconds` of CPU work (at least `frames` frames)
     t0 = time.perf_counter()
-    for _ in range(frames):
+    done = 0
+    while done < frames or time.perf_counter() - t0 < seconds:
         ob.render(bgr, xs, ys, luts, nthreads=threads)
+        done += 1
     dt = time.perf_counter() - t0
+    frames = done
     return {
         "value": round(wl.out_pixels * frames / dt / 1e6, 3),
         "unit": "Mpix/s",
@@ -359,7 +363,7 @@ def run_batched(args):
     cpu = None
     if not args.no_cpu and args.cpu_frames > 0:
         log("cpu baseline ...")
-        cpu = cpu_baseline(wl, srcs, xs, ys, ms, luts, args.cpu_frames)
+        cpu = cpu_baseline(wl, srcs, xs, ys, ms, luts, args.cpu_frames, args.cpu_seconds)
     line = base_line(METRIC, value, args.steps, args.warmup, ms_per_step, {
         "workload": f"{wl.name}: 8 cams (2x2560x1440 rect 90, 4x1920x1080 rect 100, "
                     f"2x1280x1280 fisheye 190) -> {wl.dst_w}x{wl.dst_h} equirect, "
@@ -453,7 +457,7 @@ def run_single(args, name, label, with_plain):
                                                 float_weights=True)
     line["cpu_baseline"] = None
     if not args.no_cpu and args.cpu_frames > 0:
-        line["cpu_baseline"] = cpu_baseline(wl, srcs, xs, ys, ms, luts, args.cpu_frames)
+        line["cpu_baseline"] = cpu_baseline(wl, srcs, xs, ys, ms, luts, args.cpu_frames, args.cpu_seconds)
     return line
 
 
@@ -529,12 +533,14 @@ def run_linear(args):
         wts = np.stack(wts)
         bgr = [s[..., :3].copy() for s in srcs]
         t0 = time.perf_counter()
-        for _ in range(args.cpu_frames):
+        nf = 0
+        while nf < args.cpu_frames or time.perf_counter() - t0 < args.cpu_seconds:
             O.linear_render(bgr, xs, ys, wts, luts, nthreads=threads)
+            nf += 1
         cdt = time.perf_counter() - t0
-        line["cpu_baseline"] = {"value": round(wl.out_pixels * args.cpu_frames / cdt / 1e6, 3), "unit": "Mpix/s",
+        line["cpu_baseline"] = {"value": round(wl.out_pixels * nf / cdt / 1e6, 3), "unit": "Mpix/s",
                                 "cores": threads, "kind": "port",
-                                "sample": f"{args.cpu_frames} frame(s), oracle linear-branch restatement "
+                                "sample": f"{nf} frame(s), oracle linear-branch restatement "
                                           f"(reprojection row-parallel, {threads} threads), {cdt:.1f} s"}
     return line
 
@@ -761,7 +767,9 @@ def main():
     ap.add_argument("--workload", default=None)
     ap.add_argument("--scale", type=float, default=1.0)
     ap.add_argument("--streams", type=int, default=16)
-    ap.add_argument("--cpu-frames", type=int, default=2)
+    ap.add_argument("--cpu-frames", type=int, default=1, help="at least this many CPU baseline frames")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline: whole frames until about this much CPU time")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
