@@ -235,10 +235,11 @@ def setup(name, scale):
     return wl, xs, ys, ms, srcs, luts
 
 
-def new_render(wl, xs, ys, ms):
+def new_render(wl, xs, ys, ms, tile_shape=0):
     import panostitcher_amd as P
 
     r = P.PanoramaRender()
+    r.set_tile_shape(tile_shape)
     ok = r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h), xs, ys,
                         ms, wl.levels)
     if not ok or not r.fused:
@@ -259,10 +260,23 @@ def run_batched(args):
 
     wl, xs, ys, ms, srcs, luts = setup(args.workload or "mixed8", args.scale)
     S = max(1, args.streams)
+    # down tile shape: the project's autotuner times each shape on one rig
+    shape, shape_tune = 0, None
+    if not args.no_autotune:
+        tr = P.PanoramaRender()
+        tsrc = [torch.from_numpy(x).cuda() for x in srcs]
+        tout = torch.empty((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+        shape, tms = tr.autotune_tile_shape([c.param() for c in wl.cameras],
+                                            [(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h),
+                                            wl.levels, tsrc, tout, iters=5)
+        shape_tune = {"chosen": shape, "tile": list(P.PanoramaRender.TILE_SHAPES[shape]),
+                      "ms_per_frame": dict(zip(["x".join(map(str, t)) for t in P.PanoramaRender.TILE_SHAPES], tms))}
+        del tr, tsrc, tout
+        log(f"tile shape autotune {shape_tune}")
     renders = []
     t0 = time.perf_counter()
     for _ in range(S):
-        renders.append(new_render(wl, xs, ys, ms))
+        renders.append(new_render(wl, xs, ys, ms, shape))
     r0 = renders[0]
     prep_each = (time.perf_counter() - t0) / S
     log(f"prepared {S} renders in {time.perf_counter() - t0:.1f}s, levels {r0.num_levels}, "
@@ -397,6 +411,7 @@ def run_batched(args):
     line["batched_eager"] = eager
     line["batched_graph"] = graph
     line["autotune"] = tune
+    line["tile_shape_autotune"] = shape_tune
     line["encoder_handoff"] = enc
     line["plain_dropin"] = plain
     line["plain_dropin_f32"] = plain32

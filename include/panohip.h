@@ -353,6 +353,21 @@ int pano_render_prepare_ms(const pano_render* r, double* ms4);
  * and map storage: on the device (on != 0, the default, up to 32 cameras) or
  * on the host (the reference prepares on the host; both give the same rig). */
 int pano_render_set_prepare_on_device(pano_render* r, int on);
+/* Down tile shape of the fused path (the level j -> j+1 output tile of the
+ * remap and pyrDown kernels): 0 = 64 x 16 (512 threads), 1 = 64 x 32 (1024),
+ * 2 = 128 x 16 (1024).  Takes effect at the next prepare; renders batched
+ * together need one shape.  pano_render_tile_shape: the prepared one. */
+#define PANO_TILE_SHAPES 3
+int pano_render_set_tile_shape(pano_render* r, int shape);
+int pano_render_tile_shape(const pano_render* r);
+/* Tile autotuner: prepares the rig (pano_render_prepare) with each shape,
+ * times iters frames of pano_render_render (no LUTs) on the stream after one
+ * warm-up frame, and leaves r prepared with the fastest shape.  best: the
+ * shape; ms_out (optional): PANO_TILE_SHAPES device ms per frame. */
+int pano_render_autotune_tile_shape(pano_render* r, int num, const pano_photo_param* params, const int* src_w,
+                                    const int* src_h, int dst_w, int dst_h, int max_levels,
+                                    const uint8_t* const* srcs, const size_t* src_steps, uint8_t* dst,
+                                    size_t dst_step, int iters, int* best, float* ms_out, hipStream_t stream);
 int pano_render_num_images(const pano_render* r);
 int pano_render_num_levels(const pano_render* r);
 int64_t pano_render_mem_bytes(const pano_render* r);

@@ -84,6 +84,9 @@ def lib():
             "pano_nonintersecting_masks": ([i, i, i, pp, psz, pp, psz], i),
             "pano_render_prepare_ms": ([vp, vp], i),
             "pano_render_set_prepare_on_device": ([vp, i], i),
+            "pano_render_set_tile_shape": ([vp, i], i),
+            "pano_render_tile_shape": ([vp], i),
+            "pano_render_autotune_tile_shape": ([vp, i, vp, vp, vp, i, i, i, pp, psz, vp, sz, i, vp, vp, vp], i),
             "pano_generate_map_device": ([vp, i, i, i, i, vp, vp, sz, vp, sz, vp], i),
             "pano_reproject_weighted_accumulate": ([vp, sz, i, i, vp, vp, sz, vp, sz, vp, sz, i, i, vp], i),
             "pano_reproject_params": ([vp, sz, i, i, vp, i, i, i, vp, sz, vp, sz, vp], i),
@@ -197,7 +200,8 @@ def exported_symbols():
         "pano_reproject_weighted_accumulate", "pano_reproject_params",
         "pano_load_photo_params", "pano_load_photo_params_xml", "pano_load_photo_params_pts",
         "pano_export_photo_params_xml", "pano_rotate_cameras", "pano_rotate_photo_params_xml",
-        "pano_render_set_prepare_on_device", "pano_linear_render_prepare",
+        "pano_render_set_prepare_on_device", "pano_linear_render_prepare", "pano_render_set_tile_shape",
+        "pano_render_tile_shape", "pano_render_autotune_tile_shape",
     ]
 
 
@@ -924,6 +928,36 @@ class PanoramaRender:
             return False
         _check(st, "render")
         return True
+
+    TILE_SHAPES = ((64, 16), (64, 32), (128, 16))   # down tile (width, height) per shape
+
+    def set_tile_shape(self, shape: int) -> None:
+        """Down tile shape of the next prepare (0, 1, 2: TILE_SHAPES)."""
+        _check(lib().pano_render_set_tile_shape(self._h, int(shape)), "set_tile_shape")
+
+    @property
+    def tile_shape(self) -> int:
+        return lib().pano_render_tile_shape(self._h)
+
+    def autotune_tile_shape(self, params, src_sizes, dst_size, max_levels: int, srcs, out, iters: int = 5,
+                            stream=None):
+        """Prepares with each tile shape, times `iters` frames of srcs (device BGRx
+        tensors) into out, keeps the fastest.  Returns (shape, ms per frame by shape)."""
+        n = len(params)
+        arr = (PhotoParam * n)(*params)
+        sw = np.array([s_[0] for s_ in src_sizes], np.int32)
+        shh = np.array([s_[1] for s_ in src_sizes], np.int32)
+        ptrs, steps = zip(*[_dev_ptr_step(t) for t in srcs])
+        op, ostep = _dev_ptr_step(out)
+        best = C.c_int()
+        ms = np.zeros(3, np.float32)
+        _check(lib().pano_render_autotune_tile_shape(self._h, n, C.cast(arr, C.c_void_p), sw.ctypes.data,
+                                                     shh.ctypes.data, dst_size[0], dst_size[1], max_levels,
+                                                     _ptrs(ptrs), _sizes(steps), C.c_void_p(op), ostep, int(iters),
+                                                     C.byref(best), ms.ctypes.data, _stream(stream)),
+               "autotune_tile_shape")
+        self.num_images, self.dst_size = n, tuple(dst_size)
+        return best.value, [round(float(v), 4) for v in ms]
 
     def set_prepare_on_device(self, on: bool = True) -> None:
         """Build the fused render's tables on the device (default) or the host."""

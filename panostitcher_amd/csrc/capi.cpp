@@ -215,6 +215,7 @@ struct Launch {
     int stage, level, nseg, blocks;
     const Seg* segs;
     int sched = 0;   // block -> job order (block_job)
+    int shape = 0;   // down tile shape of the renders' work lists
 };
 
 // The gain LUT pass ahead of a fused sequence: jobs over the frames of src.
@@ -240,7 +241,7 @@ int run_sequence(const std::vector<Launch>& seq, const SrcTable& st, const OutTa
             for (int k = prev + 1; k <= l.stage; k++) HIP_TRY(hipEventRecord((*ev)[k], s));
             prev = l.stage;
         }
-        HIP_TRY(launch_seg(l.stage, l.segs, l.nseg, l.blocks, l.level, st, ot, l.sched, s));
+        HIP_TRY(launch_seg(l.stage, l.segs, l.nseg, l.blocks, l.level, st, ot, l.sched, l.shape, s));
     }
     if (ev)
         for (int k = prev + 1; k <= 4; k++) HIP_TRY(hipEventRecord((*ev)[k], s));
@@ -644,7 +645,10 @@ struct DeviceRig {
             launch(kStageFinal, 0, (int)prep.fin.size());
             d_segs = upload_vec(arena, segs, err);
             if (err) return err;
-            for (size_t k = 0; k < ls.size(); k++) ls[k].segs = d_segs + k;
+            for (size_t k = 0; k < ls.size(); k++) {
+                ls[k].segs = d_segs + k;
+                ls[k].shape = prep.shape;
+            }
             seq = ls;
         }
         compute_model_bytes(have_maps, src_w, src_h);
@@ -814,6 +818,7 @@ struct pano_render {
     uint8_t* bgrx_scratch = nullptr;  // 16-bit path with encoder-plane output: the BGRx canvas
     double prep_ms[4] = {};           // last prepare: map support, seams, tables, device build
     bool prepare_on_device = true;    // fused-path tables, map storage and seams built on the device
+    int tile_shape = 0;               // down tile shape of the next prepare (kDownShapes)
     ~pano_render()
     {
         if (bgrx_scratch) (void)hipFree(bgrx_scratch);
@@ -1372,7 +1377,8 @@ int render_prepare_device(pano_render* r, int num, const int* src_w, const int* 
     if (!r->rig) return PANO_ERR_NOMEM;
     r->rig->kind = kPixU8;
     DevTables dt;
-    int err = prepare_tables_device(num, dst_h, dst_w, b.masks, b.unique, max_levels, true, r->rig->prep, dt);
+    int err = prepare_tables_device(num, dst_h, dst_w, b.masks, b.unique, max_levels, true, r->tile_shape,
+                                    r->rig->prep, dt);
     if (err) return err;
     const auto t3 = prep_clock::now();
     std::vector<size_t> steps(num, (size_t)dst_w);
@@ -1422,6 +1428,62 @@ int pano_render_set_prepare_on_device(pano_render* r, int on)
     if (!r) return PANO_ERR_ARG;
     r->prepare_on_device = on != 0;
     return PANO_OK;
+}
+
+int pano_render_set_tile_shape(pano_render* r, int shape)
+{
+    if (!r || shape < 0 || shape >= kDownShapes) return PANO_ERR_ARG;
+    r->tile_shape = shape;
+    return PANO_OK;
+}
+
+int pano_render_tile_shape(const pano_render* r)
+{
+    if (!r || !r->ok) return PANO_ERR_STATE;
+    return r->rig->prep.shape;
+}
+
+int pano_render_autotune_tile_shape(pano_render* r, int num, const pano_photo_param* params, const int* src_w,
+                                    const int* src_h, int dst_w, int dst_h, int max_levels,
+                                    const uint8_t* const* srcs, const size_t* src_steps, uint8_t* dst,
+                                    size_t dst_step, int iters, int* best, float* ms_out, hipStream_t stream)
+{
+    if (!r || !best || iters <= 0) return PANO_ERR_ARG;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return PANO_ERR_HIP;
+    }
+    int err = PANO_OK, pick = -1;
+    float best_ms = 0;
+    for (int sh = 0; sh < kDownShapes && !err; sh++) {
+        r->tile_shape = sh;
+        err = pano_render_prepare(r, num, params, src_w, src_h, dst_w, dst_h, max_levels);
+        if (!err) err = pano_render_render(r, num, srcs, src_steps, nullptr, dst, dst_step, stream);   // warm-up
+        if (!err && hipEventRecord(e0, stream) != hipSuccess) err = PANO_ERR_HIP;
+        for (int k = 0; k < iters && !err; k++)
+            err = pano_render_render(r, num, srcs, src_steps, nullptr, dst, dst_step, stream);
+        float ms = 0;
+        if (!err && (hipEventRecord(e1, stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                     hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            err = PANO_ERR_HIP;
+        if (err) break;
+        ms /= iters;
+        if (ms_out) ms_out[sh] = ms;
+        if (pick < 0 || ms < best_ms) {
+            pick = sh;
+            best_ms = ms;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err) return err;
+    *best = pick;
+    r->tile_shape = pick;
+    // leave the render prepared with the chosen shape
+    return pick == kDownShapes - 1 ? PANO_OK
+                                   : pano_render_prepare(r, num, params, src_w, src_h, dst_w, dst_h, max_levels);
 }
 
 int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const int* src_h,
@@ -1485,7 +1547,7 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
     if (err) return err;
     const auto t2 = clock::now();
     err = prepare_rig(num, dst_h, dst_w, masks, steps.data(), sp.data(), max_levels, 2, false, r->rig->kind,
-                      fused, r->rig->prep, unique.data());
+                      fused, r->rig->prep, unique.data(), fused ? r->tile_shape : 0);
     if (err) return err;
     const auto t3 = clock::now();
     err = r->rig->build(true, false, xmaps, ymaps, masks, steps.data(), src_w, src_h);
@@ -1831,6 +1893,8 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
         pano_render* r = renders[i];
         if (!r || !r->ok) return PANO_ERR_STATE;
         if (!r->rig->prep.fused0) return PANO_ERR_UNSUPPORTED;
+        // one launch per stage for all renders: one down tile shape
+        if (r->rig->prep.shape != renders[0]->rig->prep.shape) return PANO_ERR_ARG;
         cams += r->rig->prep.n;
         maxL = std::max(maxL, r->rig->prep.levels);
     }
@@ -1900,7 +1964,10 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
     if (sizeof(Seg) * flat.size() + 256 + sizeof(LutJob) * lj.size() > pano_batch::kSlotBytes)
         return PANO_ERR_UNSUPPORTED;
     b->drop_graph();
-    for (Launch& l : ls) l.sched = b->sched[l.stage];
+    for (Launch& l : ls) {
+        l.sched = b->sched[l.stage];
+        l.shape = renders[0]->rig->prep.shape;
+    }
     b->segs.swap(flat);
     b->seg_off.swap(offs);
     b->lutjobs.swap(lj);

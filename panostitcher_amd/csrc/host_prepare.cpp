@@ -195,7 +195,7 @@ void work_lists(Prepared& out, const TileValidFn& tile_has_valid, const CamHitsF
 {
     const int n = out.n, L = out.levels, rows = out.rows, cols = out.cols;
     const bool u8 = out.kind == kPixU8, fused0 = out.fused0;
-    const int dtw = u8 ? kFTW : kDownTW, dth = u8 ? kFTH : kDownTH;
+    const int dtw = u8 ? out.dtw : kDownTW, dth = u8 ? out.dth : kDownTH;
     const int ttw = u8 ? 2 * kQTW : kTileW, tth = u8 ? 2 * kQTH : kTileH;
     out.down.assign(L, std::vector<DownJob>());
     for (int j = 0; j < L; j++) {
@@ -285,14 +285,18 @@ void work_lists(Prepared& out, const TileValidFn& tile_has_valid, const CamHitsF
 
 int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps,
                 const uint8_t* const* support0, int max_levels, int min_length, bool full_boxes,
-                int kind, bool fused0, Prepared& out, const uint8_t* unique)
+                int kind, bool fused0, Prepared& out, const uint8_t* unique, int shape)
 {
+    if (shape < 0 || shape >= kDownShapes) return PANO_ERR_ARG;
     if (n <= 0 || n > 255 || rows <= 0 || cols <= 0 || !masks || !steps) return PANO_ERR_ARG;
     for (int i = 0; i < n; i++)
         if (!masks[i] || steps[i] < (size_t)cols) return PANO_ERR_ARG;
     out = Prepared();
     out.kind = kind;
     out.fused0 = fused0;
+    out.shape = kind == kPixU8 ? shape : 0;
+    out.dtw = kDownTileW[out.shape];
+    out.dth = kDownTileH[out.shape];
     out.n = n;
     out.rows = rows;
     out.cols = cols;

@@ -28,35 +28,40 @@ namespace {
 constexpr uint32_t kLo = 0x00ff00ffu;  // (B, R) lanes of a packed pixel; >> 8 gives (G, valid)
 constexpr uint32_t kHi = 0xff00ff00u;  // (G, valid) lanes in place
 
-// Down tile geometry: input region (2 kFTW + 3) x (2 kFTH + 3) at level j,
-// staged by kDT threads, kPer pixels each, with all loads of a thread issued
-// before any is used (the loops are latency-bound otherwise).
-constexpr int kIW = 2 * kFTW + 3, kIH = 2 * kFTH + 3;
-constexpr int kDT = 512;
-constexpr int kPer = (kIH * kIW + kDT - 1) / kDT;
+// Down tile geometry of shape S: output tile kTW x kTH at level j+1, input
+// region kIW x kIH = (2 kTW + 3) x (2 kTH + 3) at level j, staged by kDT
+// threads, kPer pixels each, with all loads of a thread issued before any is
+// used (the loops are latency-bound otherwise).
+template <int S>
+struct DownShape {
+    static constexpr int kTW = kDownTileW[S], kTH = kDownTileH[S], kDT = kDownThreads[S];
+    static constexpr int kIW = 2 * kTW + 3, kIH = 2 * kTH + 3;
+    static constexpr int kPer = (kIH * kIW + kDT - 1) / kDT;
+};
 
 // Horizontal 5-tap over the staged input rows, then vertical 5-tap and
 // calcDstImage; writes the (nx, ny) output tile at canvas (X, Y) of plane D.
+template <class SH>
 __device__ __forceinline__ void down_from_lds(const uint32_t* in, uint2* hs, const DownJob& J,
                                               const Plane& D)
 {
     const int ih = 2 * J.ny + 3;
-    for (int e = threadIdx.x; e < kIH * kFTW; e += kDT) {
-        int r = e / kFTW, x = e - r * kFTW;
+    for (int e = threadIdx.x; e < SH::kIH * SH::kTW; e += SH::kDT) {
+        int r = e / SH::kTW, x = e - r * SH::kTW;
         if (r >= ih || x >= J.nx) continue;
-        const uint32_t* p = in + r * kIW + 2 * x;
+        const uint32_t* p = in + r * SH::kIW + 2 * x;
         uint32_t a = p[0], b = p[1], c = p[2], d = p[3], f = p[4];
         uint32_t lo = (a & kLo) + (f & kLo) + 4u * ((b & kLo) + (d & kLo)) + 6u * (c & kLo);
         // (G, valid) in place: G sums reach bit 19, valid (0/1) sums sit at bits 24-28
         uint32_t hi = ((a & kHi) + (f & kHi) + 4u * ((b & kHi) + (d & kHi)) + 6u * (c & kHi)) >> 8;
-        hs[r * kFTW + x] = make_uint2(lo, hi);
+        hs[r * SH::kTW + x] = make_uint2(lo, hi);
     }
     __syncthreads();
-    for (int e = threadIdx.x; e < kFTH * kFTW; e += kDT) {
-        int y = e / kFTW, x = e - y * kFTW;
+    for (int e = threadIdx.x; e < SH::kTH * SH::kTW; e += SH::kDT) {
+        int y = e / SH::kTW, x = e - y * SH::kTW;
         if (y >= J.ny || x >= J.nx) continue;
-        const uint2* q = hs + (2 * y) * kFTW + x;
-        uint2 a = q[0], b = q[kFTW], c = q[2 * kFTW], d = q[3 * kFTW], f = q[4 * kFTW];
+        const uint2* q = hs + (2 * y) * SH::kTW + x;
+        uint2 a = q[0], b = q[SH::kTW], c = q[2 * SH::kTW], d = q[3 * SH::kTW], f = q[4 * SH::kTW];
         uint32_t lo = a.x + f.x + 4u * (b.x + d.x) + 6u * c.x;
         uint32_t hi = a.y + f.y + 4u * (b.y + d.y) + 6u * c.y;
         int al = (int)(hi >> 16);
@@ -172,11 +177,12 @@ __device__ __forceinline__ int lane(uint32_t v, int c) { return (int)((v >> (8 *
 
 // ------------------------------------------------------------ level 0 -> 1 --
 
-__global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ segs, int nseg,
+template <class SH>
+__global__ __launch_bounds__(SH::kDT) void k_remap_down0(const Seg* __restrict__ segs, int nseg,
                                                      SrcTable src, int sched)
 {
-    __shared__ uint32_t in[kIH * kIW];
-    __shared__ uint2 hs[kIH * kFTW];
+    __shared__ uint32_t in[SH::kIH * SH::kIW];
+    __shared__ uint2 hs[SH::kIH * SH::kTW];
     const int bid = block_job(sched);
     const Seg& S = seg_of(segs, nseg, bid);
     const RenderDesc R = ldc(S.rd);
@@ -200,21 +206,21 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
     const uint32_t mrow = (uint32_t)cam.mpitch * 8u;
     // Element e = threadIdx.x + k kDT of the region is (row, col) = (r, c);
     // kDT = kStepR kIW + kStepC, so the next slot adds (kStepR, kStepC) with a carry.
-    constexpr int kStepR = kDT / kIW, kStepC = kDT % kIW;
+    constexpr int kStepR = SH::kDT / SH::kIW, kStepC = SH::kDT % SH::kIW;
     // Software pipeline over kRounds rounds of kBatch points per thread: the
     // map points of round r+1 are in flight while round r gathers and combines.
-    constexpr int kBatch = 3, kRounds = (kPer + kBatch - 1) / kBatch;
+    constexpr int kBatch = 3, kRounds = (SH::kPer + kBatch - 1) / kBatch;
     float2 m[2][kBatch];
     int rr[kRounds * kBatch], cc[kRounds * kBatch];
     {
-        int r = threadIdx.x / kIW, c = threadIdx.x - (threadIdx.x / kIW) * kIW;
+        int r = threadIdx.x / SH::kIW, c = threadIdx.x - (threadIdx.x / SH::kIW) * SH::kIW;
 #pragma unroll
         for (int k = 0; k < kRounds * kBatch; k++) {
             rr[k] = r;
             cc[k] = c;
             r += kStepR;
             c += kStepC;
-            if (c >= kIW) { c -= kIW; r++; }
+            if (c >= SH::kIW) { c -= SH::kIW; r++; }
         }
     }
     auto fetch_map = [&](int round, float2* mm) {
@@ -241,8 +247,8 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
 #pragma unroll
         for (int b = 0; b < kBatch; b++) {
             const int k = round * kBatch + b;
-            const int e = threadIdx.x + k * kDT;
-            if (e >= kIH * kIW) continue;
+            const int e = threadIdx.x + k * SH::kDT;
+            if (e >= SH::kIH * SH::kIW) continue;
             const int r = rr[k], c = cc[k];
             const uint32_t bx = __float_as_uint(m[cur][b].x), by = __float_as_uint(m[cur][b].y);
             const float x = __builtin_fabsf(m[cur][b].x), y = __builtin_fabsf(m[cur][b].y);
@@ -264,7 +270,7 @@ __global__ __launch_bounds__(kDT) void k_remap_down0(const Seg* __restrict__ seg
         }
     }
     __syncthreads();
-    down_from_lds(in, hs, J, D);
+    down_from_lds<SH>(in, hs, J, D);
 }
 
 // Gain LUTs applied to whole source frames (transform(), GainAdjust.cpp:
@@ -304,10 +310,11 @@ __global__ __launch_bounds__(256) void k_lut_src(const LutJob* __restrict__ jobs
 
 // ------------------------------------------------------- level j -> j+1 --
 
-__global__ __launch_bounds__(kDT) void k_down_u8(const Seg* __restrict__ segs, int nseg, int sched)
+template <class SH>
+__global__ __launch_bounds__(SH::kDT) void k_down_u8(const Seg* __restrict__ segs, int nseg, int sched)
 {
-    __shared__ uint32_t in[kIH * kIW];
-    __shared__ uint2 hs[kIH * kFTW];
+    __shared__ uint32_t in[SH::kIH * SH::kIW];
+    __shared__ uint2 hs[SH::kIH * SH::kTW];
     const int bid = block_job(sched);
     const Seg& Sg = seg_of(segs, nseg, bid);
     const DownJob J = ldc((const DownJob*)Sg.jobs + (bid - Sg.begin));
@@ -316,22 +323,22 @@ __global__ __launch_bounds__(kDT) void k_down_u8(const Seg* __restrict__ segs, i
     const Plane D = ldc(planes + J.dst);
     const int bx0 = 2 * J.X - 2, by0 = 2 * J.Y - 2;
     const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
-    uint32_t v[kPer];
+    uint32_t v[SH::kPer];
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        int e = threadIdx.x + k * kDT;
-        int r = e / kIW, c = e - r * kIW;
+    for (int k = 0; k < SH::kPer; k++) {
+        int e = threadIdx.x + k * SH::kDT;
+        int r = e / SH::kIW, c = e - r * SH::kIW;
         v[k] = 0;
-        if (e < kIH * kIW && r < ih && c < iw)
+        if (e < SH::kIH * SH::kIW && r < ih && c < iw)
             v[k] = load_u32(S, bwrap(bx0 + c, S.cw), breflect(by0 + r, S.ch));
     }
 #pragma unroll
-    for (int k = 0; k < kPer; k++) {
-        int e = threadIdx.x + k * kDT;
-        if (e < kIH * kIW) in[e] = v[k];
+    for (int k = 0; k < SH::kPer; k++) {
+        int e = threadIdx.x + k * SH::kDT;
+        if (e < SH::kIH * SH::kIW) in[e] = v[k];
     }
     __syncthreads();
-    down_from_lds(in, hs, J, D);
+    down_from_lds<SH>(in, hs, J, D);
 }
 
 // ------------------------------------------------------- blend, level >= 1 --
@@ -598,16 +605,31 @@ __global__ __launch_bounds__(kQT) void k_final_u8(const Seg* __restrict__ segs, 
 
 // ------------------------------------------------------------- launchers --
 
+namespace {
+template <int S>
+void launch_down_shape(int stage, const Seg* segs, int nseg, int blocks, const SrcTable& src, int sched,
+                       hipStream_t s)
+{
+    using SH = DownShape<S>;
+    if (stage == kStageRemapDown0)
+        hipLaunchKernelGGL(k_remap_down0<SH>, dim3(blocks), dim3(SH::kDT), 0, s, segs, nseg, src, sched);
+    else
+        hipLaunchKernelGGL(k_down_u8<SH>, dim3(blocks), dim3(SH::kDT), 0, s, segs, nseg, sched);
+}
+}  // namespace
+
 hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int level,
-                      const SrcTable& src, const OutTable& outs, int sched, hipStream_t s)
+                      const SrcTable& src, const OutTable& outs, int sched, int shape, hipStream_t s)
 {
     if (blocks <= 0 || nseg <= 0) return hipSuccess;
+    static_assert(kDownShapes == 3, "one launch_down_shape per shape");
     switch (stage) {
     case kStageRemapDown0:
-        hipLaunchKernelGGL(k_remap_down0, dim3(blocks), dim3(kDT), 0, s, segs, nseg, src, sched);
-        break;
     case kStageDown:
-        hipLaunchKernelGGL(k_down_u8, dim3(blocks), dim3(kDT), 0, s, segs, nseg, sched);
+        if (shape == 0) launch_down_shape<0>(stage, segs, nseg, blocks, src, sched, s);
+        else if (shape == 1) launch_down_shape<1>(stage, segs, nseg, blocks, src, sched, s);
+        else if (shape == 2) launch_down_shape<2>(stage, segs, nseg, blocks, src, sched, s);
+        else return hipErrorInvalidValue;
         break;
     case kStageBlend:
         hipLaunchKernelGGL(k_blend_u8, dim3(blocks), dim3(kQT), 0, s, segs, nseg, level, sched);

@@ -29,8 +29,14 @@ constexpr int kRemapTW = 64, kRemapTH = 4;
 
 // 8-bit fused path: output tile (level j+1) of the down kernels, and quad
 // tiles of the blend kernels (kQTW x kQTH quads = 2 kQTW x 2 kQTH pixels,
-// one 2x2 quad per thread).
-constexpr int kFTW = 64, kFTH = 16;
+// one 2x2 quad per thread).  The down tile is one of kDownShapes shapes,
+// chosen per rig at prepare (shape 0 by default, or by the tile autotuner):
+// kDownTileW[s] x kDownTileH[s] outputs, kDownThreads[s] threads per block.
+constexpr int kDownShapes = 3;
+constexpr int kDownTileW[kDownShapes] = {64, 64, 128};
+constexpr int kDownTileH[kDownShapes] = {16, 32, 16};
+constexpr int kDownThreads[kDownShapes] = {512, 1024, 1024};
+constexpr int kFTW = kDownTileW[0], kFTH = kDownTileH[0];
 constexpr int kQTW = 64, kQTH = 4;
 constexpr int kMaxStage = 8;    // owner cameras staged in LDS at once (level 0)
 

@@ -57,6 +57,8 @@ struct Prepared {
     int max_tile_cams = 0;
     int kind = kPixU8;                           // tile geometry of the work lists
     bool fused0 = false;                         // down[0] jobs carry camera indices (8-bit render)
+    int shape = 0;                               // down tile shape (8-bit path), kDownShapes
+    int dtw = kFTW, dth = kFTH;                  // its output tile
 };
 
 // cv::distanceTransform(mask, dst, CV_DIST_L1, 3) semantics (OpenCV 3.0
@@ -97,8 +99,9 @@ void work_lists(Prepared& out, const TileValidFn& tile_has_valid, const CamHitsF
 // tiles; kPixS16: generic kernels); fused0: the 8-bit render path, whose
 // level 0 -> 1 jobs address cameras instead of planes.
 // unique (optional): the seams already computed (n planes rows x cols).
+// shape: the down tile shape of the 8-bit path (kDownShapes).
 int prepare_rig(int n, int rows, int cols, const uint8_t* const* masks, const size_t* steps,
                 const uint8_t* const* support0, int max_levels, int min_length, bool full_boxes,
-                int kind, bool fused0, Prepared& out, const uint8_t* unique = nullptr);
+                int kind, bool fused0, Prepared& out, const uint8_t* unique = nullptr, int shape = 0);
 
 }  // namespace pano

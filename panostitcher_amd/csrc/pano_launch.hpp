@@ -42,9 +42,10 @@ hipError_t launch_transform(const uint8_t* src, size_t src_step, uint8_t* dst, s
 // Fused 8-bit path (kernels_u8.hip): one launch of a stage over the segments
 // of a batch (blocks = sum of the segments' job counts).
 enum Stage : int { kStageRemapDown0 = 0, kStageDown = 1, kStageBlend = 2, kStageFinal = 3 };
-// sched: block -> job order (block_job): 0 in order, 1 contiguous per XCD.
+// sched: block -> job order (block_job): 0 in order, 1 contiguous per XCD;
+// shape: the down tile shape the jobs of the down stages were laid out for.
 hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int level,
-                      const SrcTable& src, const OutTable& outs, int sched, hipStream_t s);
+                      const SrcTable& src, const OutTable& outs, int sched, int shape, hipStream_t s);
 
 // Gain LUT pass over source frames ahead of k_remap_down0 (blocks = the
 // jobs' block total; src holds the frames the jobs' src fields index).
@@ -144,7 +145,7 @@ struct DevTables {
 // cols), n <= 32; synchronous.  Fills out as prepare_rig(..., kPixU8, fused0)
 // does, except out.unique / label / weight / wsum, which stay on the device.
 int prepare_tables_device(int n, int rows, int cols, const uint8_t* masks, const uint8_t* unique, int max_levels,
-                          bool with_cover, Prepared& out, DevTables& dev);
+                          bool with_cover, int shape, Prepared& out, DevTables& dev);
 // Margined fused map storage of camera cam (box g at level 0) from device maps
 // and masks (pitch cols) and the owner plane.
 hipError_t launch_pack_map(const float* xm, const float* ym, const uint8_t* mask, const uint8_t* label, int cols,

@@ -270,14 +270,18 @@ DevTables::~DevTables()
 }
 
 int prepare_tables_device(int n, int rows, int cols, const uint8_t* masks, const uint8_t* unique, int max_levels,
-                          bool with_cover, Prepared& out, DevTables& dev)
+                          bool with_cover, int shape, Prepared& out, DevTables& dev)
 {
     if (n <= 0 || n > 32 || rows <= 0 || cols <= 0 || !masks || !unique) return PANO_ERR_ARG;
+    if (shape < 0 || shape >= kDownShapes) return PANO_ERR_ARG;
     const int L = true_num_levels(cols, rows, max_levels, 2);
     if (L < 0) return PANO_ERR_ARG;
     out = Prepared();
     out.kind = kPixU8;
     out.fused0 = true;
+    out.shape = shape;
+    out.dtw = kDownTileW[shape];
+    out.dth = kDownTileH[shape];
     out.n = n;
     out.rows = rows;
     out.cols = cols;
@@ -323,7 +327,7 @@ int prepare_tables_device(int n, int rows, int cols, const uint8_t* masks, const
     out.full = !uncovered;
 
     // plane offsets, tile grids
-    const int dtw = kFTW, dth = kFTH, ttw = 2 * kQTW, tth = 2 * kQTH;
+    const int dtw = out.dtw, dth = out.dth, ttw = 2 * kQTW, tth = 2 * kQTH;
     std::vector<DevGeom> dg((size_t)n * (L + 1));
     size_t elems = 0;
     int nflags = 0;

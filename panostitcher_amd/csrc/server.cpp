@@ -103,10 +103,14 @@ struct pano_server {
             });
             Batch bt;
             std::vector<pano_render*> rs;
-            int cams = 0;
+            int cams = 0, shape = -1;
             for (int id : pick) {
                 Stream& s = streams[id];
                 if ((int)rs.size() == kMaxBatch || cams + s.n > kMaxSrc) break;
+                // one launch per stage needs one down tile shape: others wait for a later batch
+                const int sh = pano_render_tile_shape(s.r);
+                if (shape < 0) shape = sh;
+                else if (sh != shape) continue;
                 rs.push_back(s.r);
                 cams += s.n;
                 bt.frames.push_back(std::move(s.pending.front()));

@@ -117,3 +117,50 @@ def test_device_prepare_edge_rigs(case):
         imgs.append((from_dev(out), [r.model_bytes(k) for k in range(5)]))
     np.testing.assert_array_equal(imgs[1][0], imgs[0][0])
     assert imgs[1][1] == imgs[0][1]
+
+
+@pytest.mark.parametrize("shape", [1, 2])
+def test_tile_shapes_match_default(shape):
+    # the down tile shapes (64x32, 128x16) give the default shape's canvas bit
+    # for bit (the default's canvas is bit-exact against the oracle), with the
+    # tables built on the device and on the host alike
+    wl = synth.workload("mixed8", 0.2)
+    xs, ys, ms = synth.maps_and_masks(wl)
+    srcs = [to_dev(synth.bgrx(f)) for f in synth.frames(wl)]
+    luts = synth.luts(wl)
+    sizes = [(c.width, c.height) for c in wl.cameras]
+    imgs = []
+    for sh, on in ((0, True), (shape, True), (shape, False)):
+        note(f"tile shape {sh} on_device={on}")
+        r = P.PanoramaRender()
+        r.set_prepare_on_device(on)
+        r.set_tile_shape(sh)
+        assert r.prepare_maps(sizes, (wl.dst_w, wl.dst_h), xs, ys, ms, wl.levels)
+        assert r.tile_shape == sh
+        imgs.append(_render(r, wl, srcs, luts, False))
+    np.testing.assert_array_equal(imgs[1], imgs[0])
+    np.testing.assert_array_equal(imgs[2], imgs[0])
+
+
+def test_tile_autotune_and_batch_shapes():
+    wl = synth.workload("rect6", 0.25)
+    srcs = [to_dev(synth.bgrx(f)) for f in synth.frames(wl)]
+    params = [c.param() for c in wl.cameras]
+    sizes = [(c.width, c.height) for c in wl.cameras]
+    ref = P.PanoramaRender()
+    assert ref.prepare(params, sizes, (wl.dst_w, wl.dst_h), wl.levels)
+    want = _render(ref, wl, srcs, None, False)
+    r = P.PanoramaRender()
+    out = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+    note("tile autotune rect6@0.25")
+    best, ms = r.autotune_tile_shape(params, sizes, (wl.dst_w, wl.dst_h), wl.levels, srcs, out, iters=2)
+    note(f"best shape {best}, ms {ms}")
+    assert best in (0, 1, 2) and r.tile_shape == best and all(v > 0 for v in ms)
+    np.testing.assert_array_equal(_render(r, wl, srcs, None, False), want)
+    # renders of different shapes do not share a batched launch
+    other = P.PanoramaRender()
+    other.set_tile_shape((best + 1) % 3)
+    assert other.prepare(params, sizes, (wl.dst_w, wl.dst_h), wl.levels)
+    b = P.Batch()
+    with pytest.raises(P.PanoError):
+        b.set([r, other])
