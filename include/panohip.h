@@ -355,9 +355,10 @@ int pano_render_prepare_ms(const pano_render* r, double* ms4);
 int pano_render_set_prepare_on_device(pano_render* r, int on);
 /* Down tile shape of the fused path (the level j -> j+1 output tile of the
  * remap and pyrDown kernels): 0 = 64 x 16 (512 threads), 1 = 64 x 32 (1024),
- * 2 = 128 x 16 (1024).  Takes effect at the next prepare; renders batched
- * together need one shape.  pano_render_tile_shape: the prepared one. */
-#define PANO_TILE_SHAPES 3
+ * 2 = 128 x 16 (1024), 3 = 32 x 32 (512), 4 = 64 x 8 (256).  Takes effect at
+ * the next prepare; renders batched together need one shape.
+ * pano_render_tile_shape: the prepared one. */
+#define PANO_TILE_SHAPES 5
 int pano_render_set_tile_shape(pano_render* r, int shape);
 int pano_render_tile_shape(const pano_render* r);
 /* Tile autotuner: prepares the rig (pano_render_prepare) with each shape,

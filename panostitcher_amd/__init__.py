@@ -929,10 +929,10 @@ class PanoramaRender:
         _check(st, "render")
         return True
 
-    TILE_SHAPES = ((64, 16), (64, 32), (128, 16))   # down tile (width, height) per shape
+    TILE_SHAPES = ((64, 16), (64, 32), (128, 16), (32, 32), (64, 8))   # down tile (width, height) per shape
 
     def set_tile_shape(self, shape: int) -> None:
-        """Down tile shape of the next prepare (0, 1, 2: TILE_SHAPES)."""
+        """Down tile shape of the next prepare (an index of TILE_SHAPES)."""
         _check(lib().pano_render_set_tile_shape(self._h, int(shape)), "set_tile_shape")
 
     @property
@@ -950,7 +950,7 @@ class PanoramaRender:
         ptrs, steps = zip(*[_dev_ptr_step(t) for t in srcs])
         op, ostep = _dev_ptr_step(out)
         best = C.c_int()
-        ms = np.zeros(3, np.float32)
+        ms = np.zeros(len(self.TILE_SHAPES), np.float32)
         _check(lib().pano_render_autotune_tile_shape(self._h, n, C.cast(arr, C.c_void_p), sw.ctypes.data,
                                                      shh.ctypes.data, dst_size[0], dst_size[1], max_levels,
                                                      _ptrs(ptrs), _sizes(steps), C.c_void_p(op), ostep, int(iters),

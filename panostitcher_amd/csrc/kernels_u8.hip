@@ -622,14 +622,18 @@ hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int leve
                       const SrcTable& src, const OutTable& outs, int sched, int shape, hipStream_t s)
 {
     if (blocks <= 0 || nseg <= 0) return hipSuccess;
-    static_assert(kDownShapes == 3, "one launch_down_shape per shape");
+    static_assert(kDownShapes == 5, "one launch_down_shape per shape");
     switch (stage) {
     case kStageRemapDown0:
     case kStageDown:
-        if (shape == 0) launch_down_shape<0>(stage, segs, nseg, blocks, src, sched, s);
-        else if (shape == 1) launch_down_shape<1>(stage, segs, nseg, blocks, src, sched, s);
-        else if (shape == 2) launch_down_shape<2>(stage, segs, nseg, blocks, src, sched, s);
-        else return hipErrorInvalidValue;
+        switch (shape) {
+        case 0: launch_down_shape<0>(stage, segs, nseg, blocks, src, sched, s); break;
+        case 1: launch_down_shape<1>(stage, segs, nseg, blocks, src, sched, s); break;
+        case 2: launch_down_shape<2>(stage, segs, nseg, blocks, src, sched, s); break;
+        case 3: launch_down_shape<3>(stage, segs, nseg, blocks, src, sched, s); break;
+        case 4: launch_down_shape<4>(stage, segs, nseg, blocks, src, sched, s); break;
+        default: return hipErrorInvalidValue;
+        }
         break;
     case kStageBlend:
         hipLaunchKernelGGL(k_blend_u8, dim3(blocks), dim3(kQT), 0, s, segs, nseg, level, sched);

@@ -32,10 +32,10 @@ constexpr int kRemapTW = 64, kRemapTH = 4;
 // one 2x2 quad per thread).  The down tile is one of kDownShapes shapes,
 // chosen per rig at prepare (shape 0 by default, or by the tile autotuner):
 // kDownTileW[s] x kDownTileH[s] outputs, kDownThreads[s] threads per block.
-constexpr int kDownShapes = 3;
-constexpr int kDownTileW[kDownShapes] = {64, 64, 128};
-constexpr int kDownTileH[kDownShapes] = {16, 32, 16};
-constexpr int kDownThreads[kDownShapes] = {512, 1024, 1024};
+constexpr int kDownShapes = 5;
+constexpr int kDownTileW[kDownShapes] = {64, 64, 128, 32, 64};
+constexpr int kDownTileH[kDownShapes] = {16, 32, 16, 32, 8};
+constexpr int kDownThreads[kDownShapes] = {512, 1024, 1024, 512, 256};
 constexpr int kFTW = kDownTileW[0], kFTH = kDownTileH[0];
 constexpr int kQTW = 64, kQTH = 4;
 constexpr int kMaxStage = 8;    // owner cameras staged in LDS at once (level 0)

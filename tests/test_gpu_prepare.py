@@ -119,9 +119,9 @@ def test_device_prepare_edge_rigs(case):
     assert imgs[1][1] == imgs[0][1]
 
 
-@pytest.mark.parametrize("shape", [1, 2])
+@pytest.mark.parametrize("shape", [1, 2, 3, 4])
 def test_tile_shapes_match_default(shape):
-    # the down tile shapes (64x32, 128x16) give the default shape's canvas bit
+    # the other down tile shapes (64x32, 128x16, 32x32, 64x8) give the default shape's canvas bit
     # for bit (the default's canvas is bit-exact against the oracle), with the
     # tables built on the device and on the host alike
     wl = synth.workload("mixed8", 0.2)
@@ -155,11 +155,11 @@ def test_tile_autotune_and_batch_shapes():
     note("tile autotune rect6@0.25")
     best, ms = r.autotune_tile_shape(params, sizes, (wl.dst_w, wl.dst_h), wl.levels, srcs, out, iters=2)
     note(f"best shape {best}, ms {ms}")
-    assert best in (0, 1, 2) and r.tile_shape == best and all(v > 0 for v in ms)
+    assert best in range(len(P.PanoramaRender.TILE_SHAPES)) and r.tile_shape == best and all(v > 0 for v in ms)
     np.testing.assert_array_equal(_render(r, wl, srcs, None, False), want)
     # renders of different shapes do not share a batched launch
     other = P.PanoramaRender()
-    other.set_tile_shape((best + 1) % 3)
+    other.set_tile_shape((best + 1) % len(P.PanoramaRender.TILE_SHAPES))
     assert other.prepare(params, sizes, (wl.dst_w, wl.dst_h), wl.levels)
     b = P.Batch()
     with pytest.raises(P.PanoError):
