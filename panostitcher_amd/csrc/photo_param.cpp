@@ -51,9 +51,11 @@ struct XNode {
 };
 
 struct XmlReader {
+    static constexpr int kMaxDepth = 64;   // deeper nesting is refused (no rig file nests so deep)
     const std::string& s;
     size_t p = 0;
     bool ok = true;
+    int depth = 0;
 
     explicit XmlReader(const std::string& src) : s(src) {}
 
@@ -173,6 +175,11 @@ struct XmlReader {
     }
     bool read_element(XNode& e)
     {
+        if (++depth > kMaxDepth) return ok = false;
+        struct Leave {
+            int& d;
+            ~Leave() { d--; }
+        } leave{depth};
         p++;   // '<'
         const size_t b = p;
         while (p < s.size() && name_char(s[p])) p++;

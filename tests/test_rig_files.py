@@ -116,6 +116,15 @@ def test_xml_errors(tmp_path):
     assert P.load_photo_param_from_xml(str(f)) == []
     f.write_text("")
     assert P.load_photo_param_from_xml(str(f)) == []
+    # pathological nesting is refused, not recursed into
+    f.write_text("<Root>" + "<a>" * 100000 + "</a>" * 100000 + "</Root>")
+    assert P.load_photo_param_from_xml(str(f)) == []
+    # attributes with '>' inside quotes, self-closing elements, CDATA text
+    f.write_text(good.replace("<POSITION>", '<POSITION id="a>b" />\n<POSITION>', 1)
+                     .replace("<HFOV>195.5</HFOV>", "<HFOV><![CDATA[195.5]]></HFOV>"))
+    assert P.load_photo_param_from_xml(str(f)) == []     # the empty POSITION lacks every field
+    f.write_text(good.replace("<HFOV>195.5</HFOV>", "<HFOV><![CDATA[195.5]]></HFOV>"))
+    assert P.load_photo_param_from_xml(str(f))[0].hfov == 195.5
 
 
 def test_export_round_trip(tmp_path):
