@@ -41,11 +41,11 @@ _lib = None
 
 
 def build(arch: str = "gfx950", jobs: int = 8) -> str:
-    """Compile libpanohip.so in-tree with hipcc (cross-compiles without a GPU)."""
+    """Compile libpanohip.so in-tree with hipcc (cross-compiles without a GPU), and
+    the C++ API's test program (tests/cpp/facade_test) over it."""
     import subprocess
 
-    subprocess.run(["make", "-s", "-C", _HERE, f"-j{jobs}", f"ARCH={arch}", "libpanohip.so"],
-                   check=True)
+    subprocess.run(["make", "-s", "-C", _HERE, f"-j{jobs}", f"ARCH={arch}", "all"], check=True)
     return LIB_PATH
 
 
