@@ -143,6 +143,8 @@ struct GpuMat {
     GpuMat() = default;
     GpuMat(int r, int c, int t) { create(r, c, t); }
     GpuMat(Size sz, int t) { create(sz.height, sz.width, t); }
+    // over caller device memory (not owned): a cv::cuda::GpuMat's data and step
+    GpuMat(int r, int c, int t, void* d, size_t s) : rows(r), cols(c), data((uint8_t*)d), step(s), type_(t) {}
     void create(int r, int c, int t)
     {
         if (r == rows && c == cols && t == type_ && data) return;
