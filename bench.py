@@ -355,10 +355,61 @@ def run_batched(args):
                 P.cvt_bgr32_to_nv12(outs[k], yps[k], uvs[k], stream)
 
         dtc = timed(then_convert, max(3, args.steps // 2), 1)
+
+        # the planes downloaded to page-locked host buffers after each step (the
+        # PCIe-inclusive rate of a task that encodes on the host)
+        hy = [torch.empty((wl.dst_h, wl.dst_w), dtype=torch.uint8, pin_memory=True) for _ in range(S)]
+        huv = [torch.empty((wl.dst_h // 2, wl.dst_w), dtype=torch.uint8, pin_memory=True) for _ in range(S)]
+
+        def with_d2h():
+            batch.render_to(stream_src, nv12, lut_list, stream)
+            for k in range(S):
+                hy[k].copy_(yps[k], non_blocking=True)
+                huv[k].copy_(uvs[k], non_blocking=True)
+
+        dtd = timed(with_d2h, max(3, args.steps // 2), 1)
+
+        # the same with the downloads on a copy stream, overlapping the next
+        # step's render (two sets of device planes, events between the streams)
+        yb = [torch.empty_like(t) for t in yps]
+        uvb = [torch.empty_like(t) for t in uvs]
+        sets = [(yps, uvs, nv12), (yb, uvb, [P.Output.nv12(y, uv) for y, uv in zip(yb, uvb)])]
+        copy_stream = torch.cuda.Stream()   # default priority
+        copied = [None, None]
+        state = {"k": 0}
+
+        def overlapped():
+            cur = state["k"] & 1
+            state["k"] += 1
+            ys_, uvs_, outs_ = sets[cur]
+            if copied[cur] is not None:
+                stream.wait_event(copied[cur])      # the set's previous download is done
+            batch.render_to(stream_src, outs_, lut_list, stream)
+            rendered = torch.cuda.Event()
+            rendered.record(stream)
+            copy_stream.wait_event(rendered)
+            with torch.cuda.stream(copy_stream):
+                for k in range(S):
+                    hy[k].copy_(ys_[k], non_blocking=True)
+                    huv[k].copy_(uvs_[k], non_blocking=True)
+            copied[cur] = torch.cuda.Event()
+            copied[cur].record(copy_stream)
+
+        dto = timed(overlapped, max(3, args.steps // 2), 2)
         enc = {"nv12_fused": {"value": round(S * wl.out_pixels * args.steps / dtn / 1e6, 2), "unit": "Mpix/s",
                               "ms_per_step": round(dtn * 1e3 / args.steps, 4)},
                "bgrx_then_nv12": {"value": round(S * wl.out_pixels * max(3, args.steps // 2) / dtc / 1e6, 2),
-                                  "unit": "Mpix/s", "ms_per_step": round(dtc * 1e3 / max(3, args.steps // 2), 4)}}
+                                  "unit": "Mpix/s", "ms_per_step": round(dtc * 1e3 / max(3, args.steps // 2), 4)},
+               "nv12_fused_then_d2h_pinned": {
+                   "value": round(S * wl.out_pixels * max(3, args.steps // 2) / dtd / 1e6, 2), "unit": "Mpix/s",
+                   "ms_per_step": round(dtd * 1e3 / max(3, args.steps // 2), 4),
+                   "host_GBps": round(S * wl.out_pixels * 1.5 * max(3, args.steps // 2) / dtd / 1e9, 2),
+                   "note": "PCIe-inclusive: NV12 planes copied to page-locked host memory each step"},
+               "nv12_fused_d2h_overlapped": {
+                   "value": round(S * wl.out_pixels * max(3, args.steps // 2) / dto / 1e6, 2), "unit": "Mpix/s",
+                   "ms_per_step": round(dto * 1e3 / max(3, args.steps // 2), 4),
+                   "host_GBps": round(S * wl.out_pixels * 1.5 * max(3, args.steps // 2) / dto / 1e9, 2),
+                   "note": "downloads on a copy stream overlapping the next step's render"}}
         log(f"encoder hand-off {enc}")
 
     ms_per_step = elapsed * 1e3 / args.steps

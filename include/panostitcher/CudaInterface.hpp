@@ -24,6 +24,7 @@
 #include <cstdint>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -174,6 +175,7 @@ struct GpuMat {
     bool empty() const { return !data || rows <= 0 || cols <= 0; }
     Size size() const { return Size(cols, rows); }
     size_t elemSize() const { return elem_size(type_); }
+    long useCount() const { return own_.use_count(); }
     void upload(const Mat& m, Stream& s = Stream::Null())
     {
         create(m.rows, m.cols, m.type());
@@ -198,6 +200,114 @@ private:
     int type_ = -1;
     std::shared_ptr<uint8_t> own_;
 };
+
+// Page-locked host image (cv::cuda::HostMem, PAGE_LOCKED): the target of
+// asynchronous downloads.  Copies share the buffer; useCount() counts them.
+struct HostMem {
+    int rows = 0, cols = 0;
+    uint8_t* data = nullptr;
+    size_t step = 0;
+
+    HostMem() = default;
+    HostMem(int r, int c, int t) { create(r, c, t); }
+    void create(int r, int c, int t)
+    {
+        if (r == rows && c == cols && t == type_ && data) return;
+        const size_t es = elem_size(t);
+        if (r <= 0 || c <= 0 || !es) throw Exception(PANO_ERR_ARG, "HostMem::create");
+        void* p = nullptr;
+        if (hipHostMalloc(&p, (size_t)r * c * es, hipHostMallocDefault) != hipSuccess)
+            throw Exception(PANO_ERR_NOMEM, "HostMem::create");
+        own_.reset((uint8_t*)p, [](uint8_t* q) { (void)hipHostFree(q); });
+        rows = r;
+        cols = c;
+        type_ = t;
+        step = (size_t)c * es;
+        data = (uint8_t*)p;
+    }
+    int type() const { return type_; }
+    bool empty() const { return !data; }
+    Size size() const { return Size(cols, rows); }
+    long useCount() const { return own_.use_count(); }
+    Mat createMatHeader() const { return Mat(rows, cols, type_, data, step); }
+
+private:
+    int type_ = -1;
+    std::shared_ptr<uint8_t> own_;
+};
+
+// Asynchronous download into page-locked memory (GpuMat::download(HostMem&, Stream&)).
+inline void download(const GpuMat& src, HostMem& dst, Stream& stream)
+{
+    dst.create(src.rows, src.cols, src.type());
+    check(hipMemcpy2DAsync(dst.data, dst.step, src.data, src.step, (size_t)src.cols * src.elemSize(), src.rows,
+                           hipMemcpyDeviceToHost, stream.handle()) == hipSuccess
+              ? PANO_OK
+              : PANO_ERR_HIP,
+          "download");
+}
+
+// PinnedMemoryPool / GpuMemoryPool (Task/PinnedMemoryPool.h:7-175): buffers of
+// one shape; get() hands out a buffer no one else holds (the pool's copy is its
+// only owner) or a new one.  Thread-safe.
+template <class Buf>
+class BufferPool {
+public:
+    bool init(int rows, int cols, int type)
+    {
+        clear();
+        std::lock_guard<std::mutex> lock(mtx_);
+        try {
+            Buf test(rows, cols, type);
+        } catch (const Exception&) {
+            return false;
+        }
+        rows_ = rows;
+        cols_ = cols;
+        type_ = type;
+        init_ = true;
+        return true;
+    }
+    void clear()
+    {
+        std::lock_guard<std::mutex> lock(mtx_);
+        pool_.clear();
+    }
+    bool get(Buf& mem)
+    {
+        std::lock_guard<std::mutex> lock(mtx_);
+        mem = Buf();
+        if (!init_) return false;
+        for (Buf& b : pool_)
+            if (use_count(b) == 1) {
+                mem = b;
+                return true;
+            }
+        try {
+            Buf fresh(rows_, cols_, type_);
+            pool_.push_back(fresh);
+            mem = fresh;
+        } catch (const Exception&) {
+            return false;
+        }
+        return true;
+    }
+    size_t size()
+    {
+        std::lock_guard<std::mutex> lock(mtx_);
+        return pool_.size();
+    }
+
+private:
+    static long use_count(const HostMem& b) { return b.useCount(); }
+    static long use_count(const GpuMat& b) { return b.useCount(); }
+    std::mutex mtx_;
+    std::vector<Buf> pool_;
+    int rows_ = 0, cols_ = 0, type_ = 0;
+    bool init_ = false;
+};
+using PinnedMemoryPool = BufferPool<HostMem>;
+using GpuMemoryPool = BufferPool<GpuMat>;
 
 namespace detail {
 template <class M>

@@ -10,6 +10,7 @@
 //   facade_test gpu <rigs dir>    the full flow on the GPU
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -157,6 +158,33 @@ static void gpu_checks(const std::string& rigs)
     cvtNV12ToBGR32(yp, uv, back, stream);
     stream.waitForCompletion();
     EXPECT(back.size() == dst && uv.rows == dst.height / 2);
+
+    // encoder planes downloaded asynchronously into pooled page-locked buffers
+    // (PinnedMemoryPool, as the stitch-to-disk task does)
+    PinnedMemoryPool ypool, uvpool;
+    EXPECT(ypool.init(dst.height, dst.width, kType8UC1) && uvpool.init(dst.height / 2, dst.width, kType8UC1));
+    const uint8_t* first = nullptr;
+    {
+        HostMem hy, huv;
+        EXPECT(ypool.get(hy) && uvpool.get(huv));
+        first = hy.data;
+        HostMem other;
+        EXPECT(ypool.get(other) && other.data != hy.data);   // hy is held: a second buffer
+        download(yp, hy, stream);
+        download(uv, huv, stream);
+        stream.waitForCompletion();
+        Mat ys, uvs;
+        yp.download(ys);
+        uv.download(uvs);
+        long d = 0;
+        for (int y = 0; y < ys.rows; y++)
+            d += std::memcmp(ys.ptr(y), hy.createMatHeader().ptr(y), ys.cols) != 0;
+        for (int y = 0; y < uvs.rows; y++)
+            d += std::memcmp(uvs.ptr(y), huv.createMatHeader().ptr(y), uvs.cols) != 0;
+        EXPECT(d == 0);
+    }
+    HostMem again;
+    EXPECT(ypool.get(again) && again.data == first && ypool.size() == 2);   // released: reused
 
     // error conventions
     std::vector<Mat> bad = masks;
