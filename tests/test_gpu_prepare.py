@@ -164,3 +164,36 @@ def test_tile_autotune_and_batch_shapes():
     b = P.Batch()
     with pytest.raises(P.PanoError):
         b.set([r, other])
+
+
+def test_server_with_mixed_tile_shapes():
+    # streams whose rigs were prepared with different down tile shapes go
+    # through the serving loop in separate batches; every frame equals the
+    # frame rendered alone
+    rigs = []
+    for name, scale, shape in (("rect6", 0.2, 0), ("rect6", 0.2, 3), ("fisheye2", 0.25, 4)):
+        wl = synth.workload(name, scale)
+        xs, ys, ms = synth.maps_and_masks(wl)
+        r = P.PanoramaRender()
+        r.set_tile_shape(shape)
+        assert r.prepare_maps([(c.width, c.height) for c in wl.cameras], (wl.dst_w, wl.dst_h), xs, ys, ms,
+                              wl.levels)
+        srcs = [to_dev(synth.bgrx(s)) for s in synth.frames(wl, 1)]
+        alone = _render(r, wl, srcs, None, False)
+        rigs.append((wl, r, srcs, alone))
+    srv = P.Server(sum(r.mem_bytes() for _, r, _, _ in rigs) * 2, max_inflight=2)
+    ids = [srv.admit(r) for _, r, _, _ in rigs]
+    assert all(i is not None for i in ids)
+    note("server with tile shapes 0, 3, 4")
+    outs = []
+    for rep in range(2):
+        for k, (wl, r, srcs, _) in enumerate(rigs):
+            o = torch.zeros((wl.dst_h, wl.dst_w, 4), dtype=torch.uint8, device="cuda")
+            outs.append((k, o))
+            srv.submit(ids[k], srcs, o, None, tag=rep)
+    srv.drain()
+    done = srv.poll(64)
+    assert len(done) == len(outs) and all(d[3] == 0 for d in done)
+    srv.close()
+    for k, o in outs:
+        np.testing.assert_array_equal(from_dev(o), rigs[k][3])
