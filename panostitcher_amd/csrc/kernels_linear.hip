@@ -113,15 +113,17 @@ __global__ __launch_bounds__(256) void k_linear_render(const LinTile* __restrict
             const int Y = Y0 + 4 * r;
             wv[r] = Y < h ? ldg(C.w + (size_t)Y * C.pitch + X) : 0.f;
         }
+        // map points in flight with the weights (tiles list a camera only where
+        // its weight is non-zero somewhere, so few of these are wasted)
 #pragma unroll
         for (int r = 0; r < R; r++)
-            m[r] = wv[r] != 0.f ? ldg_f2(C.map + (size_t)(Y0 + 4 * r) * C.pitch + X) : make_float2(-1.f, -1.f);
+            m[r] = Y0 + 4 * r < h ? ldg_f2(C.map + (size_t)(Y0 + 4 * r) * C.pitch + X) : make_float2(-1.f, -1.f);
         bool ok[R];
         uint32_t p[R][4];
 #pragma unroll
         for (int r = 0; r < R; r++) {
             // a zero weight adds nothing; an invalid map point adds nothing
-            ok[r] = m[r].x >= 0.f && m[r].y >= 0.f && m[r].x < (float)C.sw && m[r].y < (float)C.sh;
+            ok[r] = wv[r] != 0.f && m[r].x >= 0.f && m[r].y >= 0.f && m[r].x < (float)C.sw && m[r].y < (float)C.sh;
             if (ok[r]) {
                 uint32_t o[4];
                 tap_offsets_fast(sstep, C.sw, C.sh, m[r].x, m[r].y, o);
@@ -207,14 +209,17 @@ __global__ __launch_bounds__(256) void k_interleave_map(const float* __restrict_
     out[(size_t)Y * dp + X] = make_float2(ldg(xm + q), ldg(ym + q));
 }
 
-// hit[(tile) n + i] = 1 for the kLinTW x kLinTH canvas tiles where mask i
-// (n planes of pitch w, plane bytes apart) is set; hit zero on entry.
-__global__ __launch_bounds__(256) void k_lin_tile_hits(const uint8_t* __restrict__ masks, size_t plane, int n, int w,
-                                                       int h, int tw, uint8_t* __restrict__ hit)
+// hit[(tile) n + i] = 1 for the kLinTW x kLinTH canvas tiles where camera i's
+// weight (n float planes of the given pitch, plane elements apart) is non-zero;
+// hit zero on entry.  A camera whose weight is zero across a tile adds nothing
+// there (sample x 0 = +0 leaves the non-negative sum as it is).
+__global__ __launch_bounds__(256) void k_lin_tile_hits(const float* __restrict__ wts, int pitch, size_t plane, int n,
+                                                       int w, int h, int tw, uint8_t* __restrict__ hit)
 {
     const int X = blockIdx.x * 64 + (threadIdx.x & 63), Y = blockIdx.y * 4 + (threadIdx.x >> 6), i = blockIdx.z;
     if (X >= w || Y >= h) return;
-    if (ldg(masks + plane * i + (size_t)Y * w + X)) hit[((size_t)(Y / kLinTH) * tw + X / kLinTW) * n + i] = 1;
+    if (ldg(wts + plane * i + (size_t)Y * pitch + X) != 0.f)
+        hit[((size_t)(Y / kLinTH) * tw + X / kLinTW) * n + i] = 1;
 }
 
 namespace {
@@ -228,12 +233,12 @@ hipError_t launch_interleave_map(const float* xm, const float* ym, int sp, float
     return hipGetLastError();
 }
 
-hipError_t launch_lin_tile_hits(const uint8_t* masks, size_t plane, int n, int w, int h, int tw, uint8_t* hit,
+hipError_t launch_lin_tile_hits(const float* wts, int pitch, size_t plane, int n, int w, int h, int tw, uint8_t* hit,
                                 hipStream_t s)
 {
     dim3 g = grid_px(w, h);
     g.z = n;
-    hipLaunchKernelGGL(k_lin_tile_hits, g, dim3(256), 0, s, masks, plane, n, w, h, tw, hit);
+    hipLaunchKernelGGL(k_lin_tile_hits, g, dim3(256), 0, s, wts, pitch, plane, n, w, h, tw, hit);
     return hipGetLastError();
 }
 

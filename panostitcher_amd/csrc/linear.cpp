@@ -298,7 +298,7 @@ int linear_prepare_dev(pano_linear_render* r, int num, const int* src_w, const i
         cams[i].sw = src_w[i];
         cams[i].sh = src_h[i];
     }
-    // tiles with the cameras whose mask (which bounds the weight) meets them, in camera order
+    // tiles with the cameras whose weight is non-zero somewhere in them, in camera order
     const int tw = (dst_w + kLinTW - 1) / kLinTW, th = (dst_h + kLinTH - 1) / kLinTH;
     std::vector<uint8_t> hit((size_t)tw * th * num, 0);
     {
@@ -306,7 +306,7 @@ int linear_prepare_dev(pano_linear_render* r, int num, const int* src_w, const i
         e = dhit.alloc(hit.size());
         if (e) return e;
         L_TRY(hipMemset(dhit.p, 0, hit.size()));
-        L_TRY(launch_lin_tile_hits(dmasks, Pc, num, dst_w, dst_h, tw, dhit.as<uint8_t>(), nullptr));
+        L_TRY(launch_lin_tile_hits(r->wts.as<float>(), pitch, P, num, dst_w, dst_h, tw, dhit.as<uint8_t>(), nullptr));
         L_TRY(hipMemcpy(hit.data(), dhit.p, hit.size(), hipMemcpyDeviceToHost));
     }
     std::vector<LinTile> tiles;

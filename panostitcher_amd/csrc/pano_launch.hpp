@@ -106,7 +106,7 @@ hipError_t launch_linear_weights(bool as_int, const uint8_t* dist, int pitch, si
                                  int h, void* out, hipStream_t s);
 hipError_t launch_interleave_map(const float* xm, const float* ym, int sp, float2* out, int dp, int w, int h,
                                  hipStream_t s);
-hipError_t launch_lin_tile_hits(const uint8_t* masks, size_t plane, int n, int w, int h, int tw, uint8_t* hit,
+hipError_t launch_lin_tile_hits(const float* wts, int pitch, size_t plane, int n, int w, int h, int tw, uint8_t* hit,
                                 hipStream_t s);
 hipError_t launch_linear_render(const LinTile* tiles, int ntiles, const int* camlist, const LinCam* cams,
                                 const SrcTable& src, uint8_t* out, size_t ostep, int w, int h, hipStream_t s);
