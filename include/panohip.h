@@ -363,8 +363,9 @@ int pano_render_set_tile_shape(pano_render* r, int shape);
 int pano_render_tile_shape(const pano_render* r);
 /* Tile autotuner: prepares the rig (pano_render_prepare) with each shape,
  * times iters frames of pano_render_render (no LUTs) on the stream after one
- * warm-up frame, and leaves r prepared with the fastest shape.  best: the
- * shape; ms_out (optional): PANO_TILE_SHAPES device ms per frame. */
+ * warm-up frame, and leaves r prepared with the fastest shape (shape 0 unless
+ * another is more than 2% faster).  best: the shape; ms_out (optional):
+ * PANO_TILE_SHAPES device ms per frame. */
 int pano_render_autotune_tile_shape(pano_render* r, int num, const pano_photo_param* params, const int* src_w,
                                     const int* src_h, int dst_w, int dst_h, int max_levels,
                                     const uint8_t* const* srcs, const size_t* src_steps, uint8_t* dst,

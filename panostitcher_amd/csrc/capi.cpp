@@ -1456,7 +1456,7 @@ int pano_render_autotune_tile_shape(pano_render* r, int num, const pano_photo_pa
         return PANO_ERR_HIP;
     }
     int err = PANO_OK, pick = -1;
-    float best_ms = 0;
+    float best_ms = 0, shape0_ms = 0;
     for (int sh = 0; sh < kDownShapes && !err; sh++) {
         r->tile_shape = sh;
         err = pano_render_prepare(r, num, params, src_w, src_h, dst_w, dst_h, max_levels);
@@ -1471,6 +1471,7 @@ int pano_render_autotune_tile_shape(pano_render* r, int num, const pano_photo_pa
         if (err) break;
         ms /= iters;
         if (ms_out) ms_out[sh] = ms;
+        if (sh == 0) shape0_ms = ms;
         if (pick < 0 || ms < best_ms) {
             pick = sh;
             best_ms = ms;
@@ -1479,6 +1480,8 @@ int pano_render_autotune_tile_shape(pano_render* r, int num, const pano_photo_pa
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (err) return err;
+    // the default shape unless another is clearly faster (timing noise is ~1%)
+    if (pick != 0 && shape0_ms <= best_ms * 1.02f) pick = 0;
     *best = pick;
     r->tile_shape = pick;
     // leave the render prepared with the chosen shape
