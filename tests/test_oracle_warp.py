@@ -146,6 +146,51 @@ def test_remap_pitch_moves_the_axis():
     assert len(hit) == 1
 
 
+def test_remap_radial_polynomial_numbers():
+    # Remap.cpp:465-477: the camera-plane radius is scaled by
+    # a t^3 + b t^2 + c t + d, t = radius / (min(crop) / 2), d = 1 - (a + b + c).
+    # Equidistant fisheye, hfov 180, crop 1000: longitude 45 deg -> radius 250,
+    # t = 0.5; with a = 0.1, b = -0.05, c = 0.02 the factor is 0.0125 - 0.0125 +
+    # 0.01 + 0.93 = 0.94 (a and c swapped would give 0.97)
+    p = O.photo_param(imageType=1, hfov=180.0, alpha=0.1, beta=-0.05, gamma=0.02)
+    sx, sy = O.remap_point(p, 1000, 1000, W, H, W / 2 - 0.5 + W * 45 / 360, H / 2 - 0.5)
+    assert abs((sx - 499.5) - 250 * 0.94) < 1e-6 and abs(sy - 499.5) < 1e-6
+    # t = 1 (longitude 90 deg, radius 500) is a fixed point of any (a, b, c)
+    sx, sy = O.remap_point(p, 1000, 1000, W, H, W / 2 - 0.5 + W * 90 / 360, H / 2 - 0.5)
+    assert abs((sx - 499.5) - 500) < 1e-6
+
+
+def test_remap_shift_and_shear_numbers():
+    # Remap.cpp:479-505: after the radial step y += shiftY, x += shiftX, then
+    # x += (shearX / cropHeight) y and y += (shearY / cropWidth) x (both from the
+    # unsheared point), around the crop centre
+    base = O.photo_param(imageType=0, hfov=90.0)
+    moved = O.photo_param(imageType=0, hfov=90.0, shiftX=12.5, shiftY=-7.25)
+    for x, y in [(W / 2 - 0.5, H / 2 - 0.5), (W / 2 - 0.5 + 40, H / 2 - 0.5 - 30)]:
+        a = O.remap_point(base, 1000, 800, W, H, x, y)
+        b = O.remap_point(moved, 1000, 800, W, H, x, y)
+        assert abs(b[0] - a[0] - 12.5) < 1e-9 and abs(b[1] - a[1] + 7.25) < 1e-9
+    # 20 deg up on the optical meridian: (0, -500 tan 20) relative to the centre;
+    # shearX = 80 (g = 0.1 with cropHeight 800) moves x by 0.1 y, y stays; shearY
+    # moves y by x / cropWidth * shearY = 0
+    sheared = O.photo_param(imageType=0, hfov=90.0, shearX=80.0, shearY=50.0)
+    y_rel = -500 * math.tan(math.radians(20))
+    sx, sy = O.remap_point(sheared, 1000, 800, W, H, W / 2 - 0.5, H / 2 - 0.5 - H * 20 / 180)
+    assert abs((sx - 499.5) - 0.1 * y_rel) < 1e-6 and abs((sy - 399.5) - y_rel) < 1e-6
+    # on the equator 30 deg right: (500 tan 30, 0): x stays, y moves by 0.05 x
+    x_rel = 500 * math.tan(math.radians(30))
+    sx, sy = O.remap_point(sheared, 1000, 800, W, H, W / 2 - 0.5 + W * 30 / 360, H / 2 - 0.5)
+    assert abs((sx - 499.5) - x_rel) < 1e-6 and abs((sy - 399.5) - 0.05 * x_rel) < 1e-6
+
+
+def test_remap_roll_turns_the_image():
+    # roll 90 deg: a point 30 deg right on the equator lands on the image's
+    # vertical axis at the same distance from the centre
+    p = O.photo_param(imageType=0, hfov=90.0, roll=90.0)
+    sx, sy = O.remap_point(p, 1000, 1000, W, H, W / 2 - 0.5 + W * 30 / 360, H / 2 - 0.5)
+    assert abs(sx - 499.5) < 1e-6 and abs(abs(sy - 499.5) - 500 * math.tan(math.radians(30))) < 1e-6
+
+
 def test_map_and_mask_validity():
     p = O.photo_param(imageType=3, hfov=195.0, cropWidth=200, cropHeight=200, circleX=100,
                       circleY=100, circleR=100)
