@@ -63,34 +63,52 @@ __global__ __launch_bounds__(256) void k_gain_stats(const Seg* __restrict__ segs
 #pragma unroll
     for (int j = 0; j < kGainMaxCams; j++) abg[j] = arg[j] = acnt[j] = 0;
     uint32_t seen = 0;
-    // per thread at most kStatW kStatH / 256 = 16 pixels: every packed 16-bit sum stays below 2^16
-    for (int e = threadIdx.x; e < kStatW * kStatH; e += 256) {
-        const int r = e / kStatW, c = e - (e / kStatW) * kStatW;
-        const int vx = 2 * J.X + c, vy = 2 * J.Y + r;
-        if (r >= chgt || c >= cwid || vx >= P0.cw || vy >= P0.ch) continue;
-        const float2 m = ldg_f2((const float2*)(mbase + __umul24((uint32_t)r, mrow) + 8u * (uint32_t)c));
-        if (!(__float_as_uint(m.y) >> 31)) continue;   // outside camera i's mask
-        const float x = __builtin_fabsf(m.x), y = __builtin_fabsf(m.y);
-        uint32_t v = 0;
-        if (x < fsw && y < fsh) {
-            uint32_t o[4], px[4];
-            tap_offsets(sstep, cam.sw, cam.sh, x, y, o);
+    // per thread kStatW kStatH / 256 = 16 pixels: every packed 16-bit sum stays below 2^16.
+    // kGB pixels at a time with their map, cover and tap loads in flight together.
+    constexpr int kPerT = kStatW * kStatH / 256, kGB = 4;
+    static_assert(kPerT * 256 == kStatW * kStatH && kPerT % kGB == 0, "whole pixel groups per thread");
+#pragma unroll 1
+    for (int k0 = 0; k0 < kPerT; k0 += kGB) {
+        float2 m[kGB];
+        uint32_t cov[kGB], px[kGB][4];
+        bool in[kGB];
 #pragma unroll
-            for (int q = 0; q < 4; q++) px[q] = ldg((const uint32_t*)(sp + o[q]));
-            bool tie = false;
-            v = bilinear_fast(__builtin_amdgcn_fractf(x), __builtin_amdgcn_fractf(y), px, tie);
-            if (tie) v = bilinear_combine((double)x, (double)y, px, nullptr);
+        for (int b = 0; b < kGB; b++) {
+            const int e = threadIdx.x + 256 * (k0 + b);
+            const int r = e / kStatW, c = e - (e / kStatW) * kStatW;
+            const int vx = 2 * J.X + c, vy = 2 * J.Y + r;
+            in[b] = r < chgt && c < cwid && vx < P0.cw && vy < P0.ch;
+            m[b] = in[b] ? ldg_f2((const float2*)(mbase + __umul24((uint32_t)r, mrow) + 8u * (uint32_t)c))
+                         : make_float2(0.f, 0.f);
+            cov[b] = in[b] ? ldg(R.cover + __umul24((uint32_t)vy, (uint32_t)P0.cw) + vx) : 0u;
         }
-        const uint32_t b = v & 255u, g = (v >> 8) & 255u, rr = (v >> 16) & 255u;
-        const uint32_t bg = b | (g << 16), rg = rr | (gray_bgr(b, g, rr) << 16);
-        const uint32_t cov = ldg(R.cover + __umul24((uint32_t)vy, (uint32_t)P0.cw) + vx);
-        seen |= cov;
 #pragma unroll
-        for (int j = 0; j < kGainMaxCams; j++) {
-            const uint32_t bit = (cov >> j) & 1u;
-            abg[j] += bit * bg;
-            arg[j] += bit * rg;
-            acnt[j] += bit;
+        for (int b = 0; b < kGB; b++) {
+            in[b] = in[b] && (__float_as_uint(m[b].y) >> 31);   // inside camera i's mask
+            const float x = __builtin_fabsf(m[b].x), y = __builtin_fabsf(m[b].y);
+            px[b][0] = px[b][1] = px[b][2] = px[b][3] = 0;
+            if (in[b] && x < fsw && y < fsh) load_taps(sp, sstep, cam.sw, cam.sh, x, y, px[b]);
+        }
+#pragma unroll
+        for (int b = 0; b < kGB; b++) {
+            if (!in[b]) continue;
+            const float x = __builtin_fabsf(m[b].x), y = __builtin_fabsf(m[b].y);
+            uint32_t v = 0;
+            if (x < fsw && y < fsh) {
+                bool tie = false;
+                v = bilinear_fast(__builtin_amdgcn_fractf(x), __builtin_amdgcn_fractf(y), px[b], tie);
+                if (tie) v = bilinear_combine((double)x, (double)y, px[b], nullptr);
+            }
+            const uint32_t bb = v & 255u, g = (v >> 8) & 255u, rr = (v >> 16) & 255u;
+            const uint32_t bg = bb | (g << 16), rg = rr | (gray_bgr(bb, g, rr) << 16);
+            seen |= cov[b];
+#pragma unroll
+            for (int j = 0; j < kGainMaxCams; j++) {
+                const uint32_t bit = (cov[b] >> j) & 1u;
+                abg[j] += bit * bg;
+                arg[j] += bit * rg;
+                acnt[j] += bit;
+            }
         }
     }
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
