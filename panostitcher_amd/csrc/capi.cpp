@@ -318,7 +318,7 @@ struct DeviceRig {
         const bool gain = fused && have_maps && n <= kGainMaxCams;
         if (gain) {
             pl.add(P);
-            pl.add(sizeof(unsigned long long) * n * n * 5);
+            pl.add(sizeof(unsigned long long) * n * n * 5 * kGainSlots);
             pl.add(sizeof(double) * (3 * n + 1));
         }
         pl.add(sizeof(int) * prep.fin_owners.size());
@@ -584,10 +584,10 @@ struct DeviceRig {
                     }
                 d_cover = upload_vec(arena, cover, err);
             }
-            d_gstats = (unsigned long long*)arena.take(sizeof(unsigned long long) * n * n * 5);
+            d_gstats = (unsigned long long*)arena.take(sizeof(unsigned long long) * n * n * 5 * kGainSlots);
             d_gains = (double*)arena.take(sizeof(double) * (3 * n + 1));
             if (!d_gstats || !d_gains) return PANO_ERR_NOMEM;
-            HIP_TRY(hipMemset(d_gstats, 0, sizeof(unsigned long long) * n * n * 5));
+            HIP_TRY(hipMemset(d_gstats, 0, sizeof(unsigned long long) * n * n * 5 * kGainSlots));
             HIP_TRY(hipMemset(d_gains, 0, sizeof(double) * (3 * n + 1)));
             if (err) return err;
         }
@@ -1800,7 +1800,7 @@ int pano_render_render_autogain(pano_render* r, int num, const uint8_t* const* s
     std::vector<hipEvent_t>* ev = r->timer.begin_frame();
     if (ev) HIP_TRY(hipEventRecord((*ev)[0], stream));
     const int n = d.prep.n;
-    HIP_TRY(launch_zero(d.d_gstats, sizeof(unsigned long long) * n * n * 5, stream));
+    HIP_TRY(launch_zero(d.d_gstats, sizeof(unsigned long long) * n * n * 5 * kGainSlots, stream));
     HIP_TRY(launch_gain(d.seq[0].segs, 1, d.seq[0].blocks, d.d_rd,
                         (flags & PANO_GAIN_TINT) ? kGainTint : 0, st, stream));
     r->lut_state = 1;   // the solve set the device's "apply LUTs" byte
@@ -1832,7 +1832,13 @@ int pano_render_gain_sums(const pano_render* r, int64_t* sums)
     if (!d.d_gstats) return PANO_ERR_UNSUPPORTED;
     const int n = d.prep.n;
     HIP_TRY(hipDeviceSynchronize());
-    HIP_TRY(hipMemcpy(sums, d.d_gstats, sizeof(int64_t) * n * n * 5, hipMemcpyDeviceToHost));
+    std::vector<int64_t> slots((size_t)kGainSlots * n * n * 5);
+    HIP_TRY(hipMemcpy(slots.data(), d.d_gstats, sizeof(int64_t) * slots.size(), hipMemcpyDeviceToHost));
+    for (int t = 0; t < n * n * 5; t++) {
+        int64_t v = 0;
+        for (int k = 0; k < kGainSlots; k++) v += slots[(size_t)k * n * n * 5 + t];
+        sums[t] = v;
+    }
     return PANO_OK;
 }
 

@@ -134,7 +134,10 @@ __global__ __launch_bounds__(256) void k_gain_stats(const Seg* __restrict__ segs
     if (threadIdx.x < n * 5) {
         const uint32_t t = part[0][threadIdx.x] + part[1][threadIdx.x] + part[2][threadIdx.x] +
                            part[3][threadIdx.x];
-        if (t) atomicAdd(R.gstats + ((size_t)J.src * n) * 5 + threadIdx.x, (unsigned long long)t);
+        // one of kGainSlots copies per block: the blocks of a camera do not all
+        // meet at the same addresses
+        const size_t slot = (size_t)(blockIdx.x % kGainSlots) * n * n * 5;
+        if (t) atomicAdd(R.gstats + slot + ((size_t)J.src * n) * 5 + threadIdx.x, (unsigned long long)t);
     }
 }
 
@@ -243,12 +246,19 @@ __global__ __launch_bounds__(256) void k_gain_solve(const RenderDesc* __restrict
     __shared__ double N[M * M], I[M * M], Rr[M * M], Bb[M * M], A[M * M];
     __shared__ double b[M], y[M], g[M], rg[M], bg[M];
     __shared__ double fac[3 * M];
+    __shared__ unsigned long long sums[M * M * 5];
     const RenderDesc R = ldc(rd);
     const int n = R.ncams;
+    for (int t = threadIdx.x; t < n * n * 5; t += blockDim.x) {
+        unsigned long long v = 0;
+        for (int k = 0; k < kGainSlots; k++) v += R.gstats[(size_t)k * n * n * 5 + t];
+        sums[t] = v;
+    }
+    __syncthreads();
     if (threadIdx.x == 0) {
         for (int i = 0; i < n; i++)
             for (int j = 0; j < n; j++) {
-                const unsigned long long* s = R.gstats + ((size_t)i * n + j) * 5;
+                const unsigned long long* s = sums + ((size_t)i * n + j) * 5;
                 const double cnt = (double)s[0];
                 const double inv = s[0] ? 1. / cnt : 0.;
                 N[i * n + j] = cnt;

@@ -96,7 +96,7 @@ struct RenderDesc {
     const uint8_t* luts;         // n x 768 LUT bytes, then 1 byte: LUTs active this frame
     uint32_t* g0comp;            // owners' level-0 samples; byte 3 = owner + 1, 0 = no owner
     const uint8_t* cover;        // gain estimation: bit j = mask j covers the canvas pixel
-    unsigned long long* gstats;  // gain estimation sums [n][n][5] (count, gray, B, G, R)
+    unsigned long long* gstats;  // gain estimation sums [kGainSlots][n][n][5] (count, gray, B, G, R)
     double* gains;               // estimated gains: n exposure, n red, n blue, anchor
     int label_pitch;
     int g0pitch;
@@ -161,6 +161,7 @@ struct LutJob {
 enum TileFlags : int { kTileTop = 1 };
 
 constexpr int kGainMaxCams = 8;   // cameras of a rig with per-frame gain estimation
+constexpr int kGainSlots = 16;    // copies of the gain sums the statistics blocks spread their atomics over
 enum GainFlags : int { kGainTint = 1 };
 
 struct SrcTable {
