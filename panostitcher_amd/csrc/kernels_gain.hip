@@ -243,31 +243,43 @@ __device__ void solve_means(int n, const double* N, const double* I, double sn, 
 __global__ __launch_bounds__(256) void k_gain_solve(const RenderDesc* __restrict__ rd, int flags)
 {
     constexpr int M = kGainMaxCams;
-    __shared__ double N[M * M], I[M * M], Rr[M * M], Bb[M * M], A[M * M];
-    __shared__ double b[M], y[M], g[M], rg[M], bg[M];
+    __shared__ double N[M * M], I[M * M], Rr[M * M], Bb[M * M];
+    // one set of normal equations per solving thread: exposure, red / green, blue / green
+    __shared__ double A[3][M * M], b[3][M], y[3][M], sol[3][M];
     __shared__ double fac[3 * M];
     __shared__ unsigned long long sums[M * M * 5];
     const RenderDesc R = ldc(rd);
     const int n = R.ncams;
-    for (int t = threadIdx.x; t < n * n * 5; t += blockDim.x) {
+    const int t0 = threadIdx.x;
+    for (int t = t0; t < n * n * 5; t += blockDim.x) {
         unsigned long long v = 0;
         for (int k = 0; k < kGainSlots; k++) v += R.gstats[(size_t)k * n * n * 5 + t];
         sums[t] = v;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int i = 0; i < n; i++)
-            for (int j = 0; j < n; j++) {
-                const unsigned long long* s = sums + ((size_t)i * n + j) * 5;
-                const double cnt = (double)s[0];
-                const double inv = s[0] ? 1. / cnt : 0.;
-                N[i * n + j] = cnt;
-                I[i * n + j] = (double)s[1] * inv;
-                const double mb = (double)s[2] * inv, mg = (double)s[3] * inv, mr = (double)s[4] * inv;
-                Rr[i * n + j] = (i == j || s[0]) ? mr / mg : 0.;
-                Bb[i * n + j] = (i == j || s[0]) ? mb / mg : 0.;
-            }
-        solve_means(n, N, I, 0.01, 100, A, b, y, g);
+    // overlap counts and means, one (i, j) entry per thread
+    for (int t = t0; t < n * n; t += blockDim.x) {
+        const int i = t / n, j = t - (t / n) * n;
+        const unsigned long long* s = sums + (size_t)t * 5;
+        const double cnt = (double)s[0];
+        const double inv = s[0] ? 1. / cnt : 0.;
+        N[t] = cnt;
+        I[t] = (double)s[1] * inv;
+        const double mb = (double)s[2] * inv, mg = (double)s[3] * inv, mr = (double)s[4] * inv;
+        Rr[t] = (i == j || s[0]) ? mr / mg : 0.;
+        Bb[t] = (i == j || s[0]) ? mb / mg : 0.;
+    }
+    __syncthreads();
+    const bool tint = flags & kGainTint;
+    // one solve per wavefront, so the three run side by side on separate SIMDs
+    if (t0 == 0) solve_means(n, N, I, 0.01, 100, A[0], b[0], y[0], sol[0]);
+    if (t0 == 64 && tint) solve_means(n, N, Rr, 1, 0.05, A[1], b[1], y[1], sol[1]);
+    if (t0 == 128 && tint) solve_means(n, N, Bb, 1, 0.05, A[2], b[2], y[2], sol[2]);
+    __syncthreads();
+    if (t0 == 0) {
+        double* g = sol[0];
+        double* rg = sol[1];
+        double* bg = sol[2];
         double max_mean = -1;
         int mi = -1;
         for (int i = 0; i < n; i++)
@@ -277,12 +289,8 @@ __global__ __launch_bounds__(256) void k_gain_solve(const RenderDesc* __restrict
             }
         const double kscale = 1.0 / g[mi];
         for (int i = 0; i < n; i++) g[i] = kscale * g[i];
-        if (flags & kGainTint) {
-            solve_means(n, N, Rr, 1, 0.05, A, b, y, rg);
-            solve_means(n, N, Bb, 1, 0.05, A, b, y, bg);
-        } else {
+        if (!tint)
             for (int i = 0; i < n; i++) rg[i] = bg[i] = 1;
-        }
         for (int i = 0; i < n; i++) {
             stg(R.gains + i, g[i]);
             stg(R.gains + n + i, rg[i]);
