@@ -104,6 +104,7 @@ __global__ __launch_bounds__(256) void k_gain_stats(const Seg* __restrict__ segs
             seen |= cov[b];
 #pragma unroll
             for (int j = 0; j < kGainMaxCams; j++) {
+                if (j >= n) break;   // uniform: the rig's camera count
                 const uint32_t bit = (cov[b] >> j) & 1u;
                 abg[j] += bit * bg;
                 arg[j] += bit * rg;
