@@ -226,14 +226,12 @@ struct LutPass {
 };
 
 // Runs a fused sequence on a stream; ev (optional): events recorded before the
-// first launch and after the remap (with the LUT pass), down, blend and final
-// stages.  st: the frames the remap reads (LUT-mapped copies where lut is given).
+// first launch and after the remap, down, blend and final stages.  st: the
+// frames the remap reads; luts: the remap applies the renders' active LUTs.
 int run_sequence(const std::vector<Launch>& seq, const SrcTable& st, const OutTable& ot,
-                 hipStream_t s, std::vector<hipEvent_t>* ev, const LutPass* lut = nullptr,
-                 bool ev0_recorded = false)
+                 hipStream_t s, std::vector<hipEvent_t>* ev, bool luts = false, bool ev0_recorded = false)
 {
     if (ev && !ev0_recorded) HIP_TRY(hipEventRecord((*ev)[0], s));
-    if (lut) HIP_TRY(launch_lut(lut->jobs, lut->njobs, lut->blocks, lut->src, s));
     int prev = kStageRemapDown0;
     for (const Launch& l : seq) {
         if (ev && l.stage != prev) {
@@ -241,7 +239,7 @@ int run_sequence(const std::vector<Launch>& seq, const SrcTable& st, const OutTa
             for (int k = prev + 1; k <= l.stage; k++) HIP_TRY(hipEventRecord((*ev)[k], s));
             prev = l.stage;
         }
-        HIP_TRY(launch_seg(l.stage, l.segs, l.nseg, l.blocks, l.level, st, ot, l.sched, l.shape, s));
+        HIP_TRY(launch_seg(l.stage, l.segs, l.nseg, l.blocks, l.level, st, ot, l.sched, l.shape, luts, s));
     }
     if (ev)
         for (int k = prev + 1; k <= 4; k++) HIP_TRY(hipEventRecord((*ev)[k], s));
@@ -1715,21 +1713,11 @@ int render_args(const pano_render* r, int num, const uint8_t* const* srcs, const
     return PANO_OK;
 }
 
-// The fused sequence with the LUT pass: the remap reads the LUT-mapped copies.
+// The fused sequence with the remap applying the render's LUTs to its taps.
 int run_fused_with_luts(DeviceRig& d, const SrcTable& st, const OutTable& ot, hipStream_t stream,
                         std::vector<hipEvent_t>* ev, bool ev0_recorded)
 {
-    LutPass lp;
-    lp.jobs = d.d_lutjobs;
-    lp.njobs = (int)d.lutjobs.size();
-    lp.blocks = d.lut_blocks;
-    lp.src = st;
-    SrcTable sr = st;
-    for (int i = 0; i < d.prep.n; i++) {
-        sr.p[i] = d.lsrc[i];
-        sr.step[i] = d.lstep[i];
-    }
-    return run_sequence(d.seq, sr, ot, stream, ev, &lp, ev0_recorded);
+    return run_sequence(d.seq, st, ot, stream, ev, true, ev0_recorded);
 }
 
 }  // namespace
@@ -2062,8 +2050,7 @@ int pano_batch_render_to(pano_batch* b, const uint8_t* const* srcs, const size_t
     bool any = false;
     int e = batch_prepare(b, srcs, src_steps, luts, outs, stream, st, sr, ot, any);
     if (e) return e;
-    const LutPass lp = batch_lut_pass(b, st);
-    e = run_sequence(b->seq, sr, ot, stream, b->timer.begin_frame(), any ? &lp : nullptr);
+    e = run_sequence(b->seq, st, ot, stream, b->timer.begin_frame(), any);
     if (e) return e;
     pano_batch::Slot& sl = b->slots[b->cur];
     HIP_TRY(hipEventRecord(sl.ev, stream));
@@ -2082,9 +2069,8 @@ int pano_batch_capture_to(pano_batch* b, const uint8_t* const* srcs, const size_
     // uploads happen here, outside the capture: the graph holds kernel launches only
     int e = batch_prepare(b, srcs, src_steps, luts, outs, stream, st, sr, ot, any);
     if (e) return e;
-    const LutPass lp = batch_lut_pass(b, st);
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
-    e = run_sequence(b->seq, sr, ot, stream, nullptr, any ? &lp : nullptr);
+    e = run_sequence(b->seq, st, ot, stream, nullptr, any);
     hipGraph_t graph = nullptr;
     const hipError_t ce = hipStreamEndCapture(stream, &graph);
     if (!e && ce != hipSuccess) e = PANO_ERR_HIP;

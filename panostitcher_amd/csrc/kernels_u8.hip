@@ -177,12 +177,16 @@ __device__ __forceinline__ int lane(uint32_t v, int c) { return (int)((v >> (8 *
 
 // ------------------------------------------------------------ level 0 -> 1 --
 
-template <class SH>
+// kLut: the render's gain LUTs (transform(), GainAdjust.cpp:295-346) are applied
+// to the four taps as they arrive, when its "LUTs active" byte is set: the same
+// bytes a LUT pass over the whole frames would have produced, without that pass.
+template <class SH, bool kLut>
 __global__ __launch_bounds__(SH::kDT) void k_remap_down0(const Seg* __restrict__ segs, int nseg,
                                                      SrcTable src, int sched)
 {
     __shared__ uint32_t in[SH::kIH * SH::kIW];
     __shared__ uint2 hs[SH::kIH * SH::kTW];
+    __shared__ uint32_t lut_w[kLut ? 192 : 1];
     const int bid = block_job(sched);
     const Seg& S = seg_of(segs, nseg, bid);
     const RenderDesc R = ldc(S.rd);
@@ -198,6 +202,16 @@ __global__ __launch_bounds__(SH::kDT) void k_remap_down0(const Seg* __restrict__
     const int iw = 2 * J.nx + 3, ih = 2 * J.ny + 3;
     const float fsw = (float)cam.sw, fsh = (float)cam.sh;
     const uint32_t owner_byte = (uint32_t)(J.src + 1) << 24;   // byte 3 of g0comp: owner + 1
+    bool use_lut = false;
+    if (kLut) {
+        const uint8_t* lb = R.luts;
+        use_lut = ldg(lb + (size_t)R.ncams * 768) != 0;   // uniform over the block
+        if (use_lut) {
+            for (int e = threadIdx.x; e < 192; e += SH::kDT) lut_w[e] = ldg((const uint32_t*)(lb + cam.lut) + e);
+            __syncthreads();
+        }
+    }
+    const uint8_t* lut = (const uint8_t*)lut_w;
     // the tile's core (level-0 pixels 2X.., 2Y..) clipped to the canvas
     const uint32_t core_w = (uint32_t)min(2 * J.nx, P0.cw - 2 * J.X);
     const uint32_t core_h = (uint32_t)min(2 * J.ny, P0.ch - 2 * J.Y);
@@ -254,6 +268,13 @@ __global__ __launch_bounds__(SH::kDT) void k_remap_down0(const Seg* __restrict__
             const float x = __builtin_fabsf(m[cur][b].x), y = __builtin_fabsf(m[cur][b].y);
             uint32_t v = 0;
             if (x < fsw && y < fsh) {
+                if (kLut && use_lut)
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        const uint32_t p = px[b][q];
+                        px[b][q] = (uint32_t)lut[p & 255u] | ((uint32_t)lut[256 + ((p >> 8) & 255u)] << 8) |
+                                   ((uint32_t)lut[512 + ((p >> 16) & 255u)] << 16) | (p & 0xff000000u);
+                    }
                 bool tie = false;
                 v = bilinear_fast(__builtin_amdgcn_fractf(x), __builtin_amdgcn_fractf(y), px[b], tie);
                 if (tie) v = bilinear_combine((double)x, (double)y, px[b], nullptr);
@@ -608,18 +629,20 @@ __global__ __launch_bounds__(kQT) void k_final_u8(const Seg* __restrict__ segs, 
 namespace {
 template <int S>
 void launch_down_shape(int stage, const Seg* segs, int nseg, int blocks, const SrcTable& src, int sched,
-                       hipStream_t s)
+                       bool luts, hipStream_t s)
 {
     using SH = DownShape<S>;
-    if (stage == kStageRemapDown0)
-        hipLaunchKernelGGL(k_remap_down0<SH>, dim3(blocks), dim3(SH::kDT), 0, s, segs, nseg, src, sched);
+    if (stage == kStageRemapDown0 && luts)
+        hipLaunchKernelGGL((k_remap_down0<SH, true>), dim3(blocks), dim3(SH::kDT), 0, s, segs, nseg, src, sched);
+    else if (stage == kStageRemapDown0)
+        hipLaunchKernelGGL((k_remap_down0<SH, false>), dim3(blocks), dim3(SH::kDT), 0, s, segs, nseg, src, sched);
     else
         hipLaunchKernelGGL(k_down_u8<SH>, dim3(blocks), dim3(SH::kDT), 0, s, segs, nseg, sched);
 }
 }  // namespace
 
 hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int level,
-                      const SrcTable& src, const OutTable& outs, int sched, int shape, hipStream_t s)
+                      const SrcTable& src, const OutTable& outs, int sched, int shape, bool luts, hipStream_t s)
 {
     if (blocks <= 0 || nseg <= 0) return hipSuccess;
     static_assert(kDownShapes == 5, "one launch_down_shape per shape");
@@ -627,11 +650,11 @@ hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int leve
     case kStageRemapDown0:
     case kStageDown:
         switch (shape) {
-        case 0: launch_down_shape<0>(stage, segs, nseg, blocks, src, sched, s); break;
-        case 1: launch_down_shape<1>(stage, segs, nseg, blocks, src, sched, s); break;
-        case 2: launch_down_shape<2>(stage, segs, nseg, blocks, src, sched, s); break;
-        case 3: launch_down_shape<3>(stage, segs, nseg, blocks, src, sched, s); break;
-        case 4: launch_down_shape<4>(stage, segs, nseg, blocks, src, sched, s); break;
+        case 0: launch_down_shape<0>(stage, segs, nseg, blocks, src, sched, luts, s); break;
+        case 1: launch_down_shape<1>(stage, segs, nseg, blocks, src, sched, luts, s); break;
+        case 2: launch_down_shape<2>(stage, segs, nseg, blocks, src, sched, luts, s); break;
+        case 3: launch_down_shape<3>(stage, segs, nseg, blocks, src, sched, luts, s); break;
+        case 4: launch_down_shape<4>(stage, segs, nseg, blocks, src, sched, luts, s); break;
         default: return hipErrorInvalidValue;
         }
         break;

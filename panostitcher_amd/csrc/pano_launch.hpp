@@ -44,8 +44,11 @@ hipError_t launch_transform(const uint8_t* src, size_t src_step, uint8_t* dst, s
 enum Stage : int { kStageRemapDown0 = 0, kStageDown = 1, kStageBlend = 2, kStageFinal = 3 };
 // sched: block -> job order (block_job): 0 in order, 1 contiguous per XCD;
 // shape: the down tile shape the jobs of the down stages were laid out for.
+// luts: the remap stage applies each render's gain LUTs where its "LUTs active"
+// byte is set (the frames in src are the unmapped ones).
 hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int level,
-                      const SrcTable& src, const OutTable& outs, int sched, int shape, hipStream_t s);
+                      const SrcTable& src, const OutTable& outs, int sched, int shape, bool luts,
+                      hipStream_t s);
 
 // Gain LUT pass over source frames ahead of k_remap_down0 (blocks = the
 // jobs' block total; src holds the frames the jobs' src fields index).
