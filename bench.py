@@ -679,7 +679,7 @@ def run_autogain(args):
                           "p99": round(float(np.percentile(lat, 99)), 4), "streams": 1,
                           "budget_60fps": 16.667}
     line["roofline"] = roofline(model, per_frame, 1, own, None)
-    line["roofline"]["note"] = "remap stage time includes the gain statistics, solve and LUT pass"
+    line["roofline"]["note"] = "remap stage time includes the gain statistics and solve; the remap applies the LUTs"
     line["last_gains"] = {"exposure": [round(float(g), 5) for g in gains], "anchor": anchor}
     line["cpu_baseline"] = None
     return line

@@ -157,8 +157,6 @@ PlaneRef plane_ref(const Plane& p, int cam)
     return f;
 }
 
-// Row step of a LUT-mapped frame copy.
-size_t lut_step(int w) { return ((size_t)w * 4 + 255) / 256 * 256; }
 
 // A map point in fused storage: |x|, |y| with the flags in the sign bits, or
 // +inf for a point the remap treats as outside the source (including NaN).
@@ -218,13 +216,6 @@ struct Launch {
     int shape = 0;   // down tile shape of the renders' work lists
 };
 
-// The gain LUT pass ahead of a fused sequence: jobs over the frames of src.
-struct LutPass {
-    const LutJob* jobs = nullptr;
-    int njobs = 0, blocks = 0;
-    SrcTable src{};
-};
-
 // Runs a fused sequence on a stream; ev (optional): events recorded before the
 // first launch and after the remap, down, blend and final stages.  st: the
 // frames the remap reads; luts: the remap applies the renders' active LUTs.
@@ -275,11 +266,6 @@ struct DeviceRig {
     uint32_t* d_g0comp = nullptr;           // owner's level-0 value per canvas pixel (8-bit render)
     Seg* d_segs = nullptr;                  // render() alone: one segment per fused launch
     std::vector<Launch> seq;                // render() alone: the fused launch sequence
-    std::vector<uint8_t*> lsrc;             // fused: LUT-mapped copy of each camera's frame
-    std::vector<uint32_t> lstep;
-    std::vector<LutJob> lutjobs;            // fused: LUT pass over this render's frames
-    LutJob* d_lutjobs = nullptr;
-    int lut_blocks = 0;
     uint8_t* d_cover = nullptr;             // gain estimation: masks covering each canvas pixel
     unsigned long long* d_gstats = nullptr;
     double* d_gains = nullptr;
@@ -347,9 +333,7 @@ struct DeviceRig {
                 size_t cells = fused ? map_cells(g) : g.pixels();
                 pl.add(cells * 8);
                 if (!fused) pl.add(cells);
-                if (fused) pl.add(lut_step(src_w[i]) * (size_t)src_h[i]);
             }
-            if (fused) pl.add(sizeof(LutJob) * n);
             pl.add((size_t)n * 768 + 4);
         }
         pl.add(sizeof(Seg) * (2 * L + 2));
@@ -540,30 +524,6 @@ struct DeviceRig {
             d_luts = (uint8_t*)arena.take((size_t)n * 768 + 4);
             if (!d_luts) return PANO_ERR_NOMEM;
             HIP_TRY(hipMemset(d_luts, 0, (size_t)n * 768 + 4));
-            if (fused) {
-                lsrc.assign(n, nullptr);
-                lstep.assign(n, 0);
-                lutjobs.clear();
-                lut_blocks = 0;
-                for (int i = 0; i < n; i++) {
-                    lstep[i] = (uint32_t)lut_step(src_w[i]);
-                    lsrc[i] = (uint8_t*)arena.take((size_t)lstep[i] * src_h[i]);
-                    if (!lsrc[i]) return PANO_ERR_NOMEM;
-                    LutJob J{};
-                    J.lut = d_luts + (size_t)i * 768;
-                    J.flag = d_luts + (size_t)n * 768;
-                    J.out = lsrc[i];
-                    J.out_step = lstep[i];
-                    J.src = i;
-                    J.w = src_w[i];
-                    J.h = src_h[i];
-                    J.rows = std::max(1, 8192 / src_w[i]);
-                    J.begin = lut_blocks;
-                    lut_blocks += (J.h + J.rows - 1) / J.rows;
-                    lutjobs.push_back(J);
-                }
-                d_lutjobs = upload_vec(arena, lutjobs, err);
-            }
             if (err) return err;
         }
         if (gain) {
@@ -849,7 +809,7 @@ struct pano_render {
 // unless kSlots compositions are still in flight.
 struct pano_batch {
     static constexpr int kSlots = 4;
-    static constexpr size_t kSlotBytes = 64 << 10;   // >= 32 launches x 32 renders of Seg + 128 LutJob
+    static constexpr size_t kSlotBytes = 64 << 10;   // >= 32 launches x 32 renders of Seg
     struct Slot {
         uint8_t* dev = nullptr;
         hipEvent_t ev = nullptr;
@@ -857,11 +817,9 @@ struct pano_batch {
     };
     std::vector<pano_render*> renders;
     int total_cams = 0;
-    // host tables of the current composition: segs of every launch, then LUT jobs
+    // host tables of the current composition: segs of every launch
     std::vector<Seg> segs;
-    std::vector<LutJob> lutjobs;
     std::vector<size_t> seg_off;         // per launch: first Seg
-    int lut_blocks = 0;
     std::vector<Launch> seq;             // segs pointers into the current slot
     bool dirty = false;                  // tables not yet uploaded
     Slot slots[kSlots];
@@ -876,7 +834,7 @@ struct pano_batch {
     // captured launch sequence (pano_batch_capture) and the tables it reads
     hipGraphExec_t exec = nullptr;
     int exec_slot = -1;
-    std::vector<char> exec_luts;          // per render: captured with the LUT pass
+    std::vector<char> exec_luts;          // per render: captured with its LUTs applied
     void drop_graph()
     {
         if (!exec) return;
@@ -905,26 +863,17 @@ struct pano_batch {
     // Upload the current tables into the next slot on stream s.
     int upload(hipStream_t s)
     {
-        const size_t sb = sizeof(Seg) * segs.size(), lb = sizeof(LutJob) * lutjobs.size();
-        const size_t loff = (sb + 255) / 256 * 256;
-        if (loff + lb > kSlotBytes) return PANO_ERR_UNSUPPORTED;
+        const size_t sb = sizeof(Seg) * segs.size();
+        if (sb > kSlotBytes) return PANO_ERR_UNSUPPORTED;
         const int k = (cur + 1) % kSlots;
         Slot& sl = slots[k];
         if (sl.used) HIP_TRY(hipEventSynchronize(sl.ev));
-        std::vector<uint8_t> host(loff + lb);
-        if (sb) memcpy(host.data(), segs.data(), sb);
-        if (lb) memcpy(host.data() + loff, lutjobs.data(), lb);
-        int e = ring.upload(host.data(), host.size(), sl.dev, s);
+        int e = ring.upload(segs.data(), sb, sl.dev, s);
         if (e) return e;
         for (size_t i = 0; i < seq.size(); i++) seq[i].segs = (const Seg*)sl.dev + seg_off[i];
         cur = k;
         dirty = false;
         return PANO_OK;
-    }
-    const LutJob* lut_table() const
-    {
-        const size_t sb = sizeof(Seg) * segs.size();
-        return (const LutJob*)(slots[cur].dev + (sb + 255) / 256 * 256);
     }
 };
 
@@ -1411,7 +1360,7 @@ int render_prepare_limits(int num, const int* src_w, const int* src_h, int dst_w
     for (int i = 0; i < num; i++) {
         if (src_w[i] <= 0 || src_h[i] <= 0) return PANO_ERR_ARG;
         // 24-bit row offsets in the remap, 32-bit frame offsets
-        if (lut_step(src_w[i]) >= (1u << 24) || lut_step(src_w[i]) * (uint64_t)src_h[i] > 0xffffffffu)
+        if ((size_t)src_w[i] * 4 >= (1u << 24) || (size_t)src_w[i] * 4 * (uint64_t)src_h[i] > 0xffffffffu)
             return PANO_ERR_UNSUPPORTED;
     }
     return PANO_OK;
@@ -1509,7 +1458,7 @@ int pano_render_prepare_maps(pano_render* r, int num, const int* src_w, const in
     for (int i = 0; i < num; i++) {
         if (!xmaps[i] || !ymaps[i] || !masks[i] || src_w[i] <= 0 || src_h[i] <= 0) return PANO_ERR_ARG;
         // 24-bit row offsets in the remap, 32-bit frame offsets
-        if (lut_step(src_w[i]) >= (1u << 24) || lut_step(src_w[i]) * (uint64_t)src_h[i] > 0xffffffffu)
+        if ((size_t)src_w[i] * 4 >= (1u << 24) || (size_t)src_w[i] * 4 * (uint64_t)src_h[i] > 0xffffffffu)
             return PANO_ERR_UNSUPPORTED;
         sp[i] = support[i].data();
     }
@@ -1946,19 +1895,7 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
         offs.push_back(flat.size());
         flat.insert(flat.end(), t.begin(), t.end());
     }
-    std::vector<LutJob> lj;
-    int lblocks = 0, base = 0;
-    for (int i = 0; i < num; i++) {
-        const DeviceRig& d = *renders[i]->rig;
-        for (LutJob J : d.lutjobs) {
-            J.src += base;
-            J.begin = lblocks;
-            lblocks += (J.h + J.rows - 1) / J.rows;
-            lj.push_back(J);
-        }
-        base += d.prep.n;
-    }
-    if (sizeof(Seg) * flat.size() + 256 + sizeof(LutJob) * lj.size() > pano_batch::kSlotBytes)
+    if (sizeof(Seg) * flat.size() > pano_batch::kSlotBytes)
         return PANO_ERR_UNSUPPORTED;
     b->drop_graph();
     for (Launch& l : ls) {
@@ -1967,8 +1904,6 @@ int pano_batch_set(pano_batch* b, int num, pano_render* const* renders)
     }
     b->segs.swap(flat);
     b->seg_off.swap(offs);
-    b->lutjobs.swap(lj);
-    b->lut_blocks = lblocks;
     b->seq = ls;
     b->renders.assign(renders, renders + num);
     b->total_cams = cams;
@@ -1982,10 +1917,10 @@ namespace {
 
 // The frame and canvas tables of one batched render, after checking the
 // arguments; uploads the renders' LUTs and the batch's tables on the stream.
-// sr: the frames the remap reads (LUT-mapped copies for renders with LUTs).
+// any: whether the remap applies LUTs for some render this call.
 int batch_prepare(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
                   const uint8_t* const* luts, const pano_output* outs, hipStream_t stream, SrcTable& st,
-                  SrcTable& sr, OutTable& ot, bool& any)
+                  OutTable& ot, bool& any)
 {
     if (!b) return PANO_ERR_ARG;
     if (b->renders.empty()) return PANO_ERR_STATE;
@@ -2005,21 +1940,13 @@ int batch_prepare(pano_batch* b, const uint8_t* const* srcs, const size_t* src_s
             st.step[k] = (uint32_t)src_steps[k];
         }
     }
-    // frames of renders with LUTs this call are read from their LUT-mapped copies
-    sr = st;
+    // each render's LUTs and its "LUTs active" byte; any: the remap applies LUTs
     any = false;
-    k = 0;
     for (size_t i = 0; i < b->renders.size(); i++) {
-        pano_render* r = b->renders[i];
         const uint8_t* lt = luts ? luts[i] : nullptr;
-        int e = r->upload_luts(lt, stream);
+        int e = b->renders[i]->upload_luts(lt, stream);
         if (e) return e;
-        for (int c = 0; c < r->rig->prep.n; c++, k++)
-            if (lt) {
-                sr.p[k] = r->rig->lsrc[c];
-                sr.step[k] = r->rig->lstep[c];
-                any = true;
-            }
+        any = any || lt;
     }
     if (b->dirty) {
         int e = b->upload(stream);
@@ -2028,15 +1955,6 @@ int batch_prepare(pano_batch* b, const uint8_t* const* srcs, const size_t* src_s
     return PANO_OK;
 }
 
-LutPass batch_lut_pass(const pano_batch* b, const SrcTable& st)
-{
-    LutPass lp;
-    lp.jobs = b->lut_table();
-    lp.njobs = (int)b->lutjobs.size();
-    lp.blocks = b->lut_blocks;
-    lp.src = st;
-    return lp;
-}
 
 }  // namespace
 
@@ -2045,10 +1963,10 @@ extern "C" {
 int pano_batch_render_to(pano_batch* b, const uint8_t* const* srcs, const size_t* src_steps,
                          const uint8_t* const* luts, const pano_output* outs, hipStream_t stream)
 {
-    SrcTable st, sr;
+    SrcTable st;
     OutTable ot;
     bool any = false;
-    int e = batch_prepare(b, srcs, src_steps, luts, outs, stream, st, sr, ot, any);
+    int e = batch_prepare(b, srcs, src_steps, luts, outs, stream, st, ot, any);
     if (e) return e;
     e = run_sequence(b->seq, st, ot, stream, b->timer.begin_frame(), any);
     if (e) return e;
@@ -2063,11 +1981,11 @@ int pano_batch_capture_to(pano_batch* b, const uint8_t* const* srcs, const size_
 {
     if (!b || !stream) return PANO_ERR_ARG;   // the null stream cannot be captured
     b->drop_graph();
-    SrcTable st, sr;
+    SrcTable st;
     OutTable ot;
     bool any = false;
     // uploads happen here, outside the capture: the graph holds kernel launches only
-    int e = batch_prepare(b, srcs, src_steps, luts, outs, stream, st, sr, ot, any);
+    int e = batch_prepare(b, srcs, src_steps, luts, outs, stream, st, ot, any);
     if (e) return e;
     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
     e = run_sequence(b->seq, st, ot, stream, nullptr, any);
@@ -2122,7 +2040,7 @@ int pano_batch_replay(pano_batch* b, const uint8_t* const* luts, hipStream_t str
     if (!b->exec || b->dirty || b->cur != b->exec_slot) return PANO_ERR_STATE;
     for (size_t i = 0; i < b->renders.size(); i++) {
         const uint8_t* lt = luts ? luts[i] : nullptr;
-        if (lt && !b->exec_luts[i]) return PANO_ERR_ARG;   // captured without the LUT pass
+        if (lt && !b->exec_luts[i]) return PANO_ERR_ARG;   // captured with the LUT-free remap
         if (lt) {
             int e = b->renders[i]->upload_luts(lt, stream);
             if (e) return e;

@@ -294,41 +294,6 @@ __global__ __launch_bounds__(SH::kDT) void k_remap_down0(const Seg* __restrict__
     down_from_lds<SH>(in, hs, J, D);
 }
 
-// Gain LUTs applied to whole source frames (transform(), GainAdjust.cpp:
-// 295-346): byte c of a pixel becomes lut[256 c + byte], c = 0..2; byte 3 is kept.
-__global__ __launch_bounds__(256) void k_lut_src(const LutJob* __restrict__ jobs, int njobs,
-                                                 SrcTable src)
-{
-    __shared__ uint32_t lut_w[192];
-    int j = 0;
-    while (j + 1 < njobs && (int)blockIdx.x >= ldc(&jobs[j + 1].begin)) j++;
-    const LutJob J = ldc(jobs + j);
-    if (!ldg(J.flag)) return;   // LUTs off for this render's frame
-    if (threadIdx.x < 192) lut_w[threadIdx.x] = ldg((const uint32_t*)J.lut + threadIdx.x);
-    __syncthreads();
-    const uint8_t* L = (const uint8_t*)lut_w;
-    const uint8_t* in = src.p[J.src];
-    const uint32_t step = src.step[J.src];
-    const int y0 = ((int)blockIdx.x - J.begin) * J.rows;
-    const int y1 = min(y0 + J.rows, J.h);
-    for (int y = y0; y < y1; y++) {
-        const uint32_t* ir = (const uint32_t*)(in + (size_t)y * step);
-        uint32_t* orow = (uint32_t*)(J.out + (size_t)y * J.out_step);
-        for (int x = threadIdx.x; x < J.w; x += 4 * 256) {
-            uint32_t p[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) p[k] = x + 256 * k < J.w ? ldg(ir + x + 256 * k) : 0u;
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (x + 256 * k >= J.w) continue;
-                const uint32_t q = (uint32_t)L[p[k] & 255u] | ((uint32_t)L[256 + ((p[k] >> 8) & 255u)] << 8) |
-                                   ((uint32_t)L[512 + ((p[k] >> 16) & 255u)] << 16) | (p[k] & 0xff000000u);
-                stg(orow + x + 256 * k, q);
-            }
-        }
-    }
-}
-
 // ------------------------------------------------------- level j -> j+1 --
 
 template <class SH>
@@ -670,11 +635,5 @@ hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int leve
     return hipGetLastError();
 }
 
-hipError_t launch_lut(const LutJob* jobs, int njobs, int blocks, const SrcTable& src, hipStream_t s)
-{
-    if (blocks <= 0 || njobs <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_lut_src, dim3(blocks), dim3(256), 0, s, jobs, njobs, src);
-    return hipGetLastError();
-}
 
 }  // namespace pano

@@ -146,18 +146,6 @@ struct RemapJob { int cam, lx, ly, nx, ny, pad0, pad1, pad2; };   // box-local t
 struct DownJob { int src, dst, X, Y, nx, ny, mx, my; };
 struct TileJob { int X, Y, nx, ny, r, off, cnt, flags; };         // canvas tile at level j
 
-// Gain LUT pass of one source frame (transform(), GainAdjust.cpp:295-346):
-// blocks [begin, begin + ceil(h / rows)) map rows of SrcTable entry src into
-// out when the render's byte *flag is non-zero.  lut: 3 x 256 bytes (B, G, R),
-// 4-byte aligned.
-struct LutJob {
-    const uint8_t* lut;
-    const uint8_t* flag;
-    uint8_t* out;
-    uint32_t out_step;
-    int src, w, h, rows, begin;
-};
-
 enum TileFlags : int { kTileTop = 1 };
 
 constexpr int kGainMaxCams = 8;   // cameras of a rig with per-frame gain estimation

@@ -50,10 +50,6 @@ hipError_t launch_seg(int stage, const Seg* segs, int nseg, int blocks, int leve
                       const SrcTable& src, const OutTable& outs, int sched, int shape, bool luts,
                       hipStream_t s);
 
-// Gain LUT pass over source frames ahead of k_remap_down0 (blocks = the
-// jobs' block total; src holds the frames the jobs' src fields index).
-hipError_t launch_lut(const LutJob* jobs, int njobs, int blocks, const SrcTable& src, hipStream_t s);
-
 // Per-frame gain estimation (kernels_gain.hip): statistics over the fused
 // level-0 jobs of segs (the remap launch's table), then the solve writing the
 // render's gains and LUTs (kGainTint: also the tint ratios).  The statistics
